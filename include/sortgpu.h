@@ -1,0 +1,130 @@
+/*
+ * sortgpu.h — C ABI of the MI355X-native ray–scene intersection and path-tracing library.
+ *
+ * This is the drop-in boundary for SORT's spatial accelerator (the `Accelerator` plugin
+ * surface: Build / GetIntersect / IsOccluded) and its path-tracing integrator entry (`Li`).
+ * Every entry point takes plain pointers and sizes; no C++ or framework types cross it, and
+ * no exception crosses it: each call returns SORTGPU_OK (0) or a negative status, with a
+ * message available from sortgpu_last_error().
+ *
+ * Citations name the reference interface each entry replaces (paths relative to SORT's src/).
+ *
+ * Pointer conventions: arguments named d_* are device pointers (HBM) that the caller owns;
+ * arguments without the prefix are host pointers. `stream` is a hipStream_t (NULL = default
+ * stream); calls are stream-ordered and return without synchronising unless stated.
+ */
+#ifndef SORTGPU_H
+#define SORTGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (replaces SORT's bool returns + sAssert, core/sassert.h:27-45) ---- */
+#define SORTGPU_OK 0
+#define SORTGPU_ERR_ARG (-1)    /* invalid argument (null pointer, bad size, bad option)   */
+#define SORTGPU_ERR_HIP (-2)    /* a HIP runtime call failed                                */
+#define SORTGPU_ERR_OOM (-3)    /* device or host allocation failed                         */
+#define SORTGPU_ERR_BUDGET (-4) /* request refused by the HBM admission budget              */
+#define SORTGPU_ERR_STATE (-5)  /* object in the wrong state (e.g. query before build)      */
+
+/* ---- records ---- */
+
+/* One ray: origin, tmin, direction, tmax (32 B). Mirrors the algorithmic part of SORT's Ray
+ * (math/ray.h:37-93: m_Ori, m_Dir, m_fMin, m_fMax). Directions need not be normalised. */
+typedef struct sortgpu_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} sortgpu_ray;
+
+/* Closest hit (16 B). prim = primitive id in build order (triangles 0..ntris-1, then quads
+ * ntris..ntris+nquads-1), 0xFFFFFFFF on a miss (t = FLT_MAX). u, v = barycentric weights of
+ * vertices 1 and 2 (simd/simd_triangle.h:296-297); 0 for quads. */
+typedef struct sortgpu_hit {
+    float t;
+    uint32_t prim;
+    float u, v;
+} sortgpu_hit;
+
+/* Analytic area-light quad (shape/quad.cpp): local plane y = 0, |x| <= size_x/2,
+ * |z| <= size_y/2, normal +y; mat = light-to-world 4x4 row-major, inv = its inverse. */
+typedef struct sortgpu_quad {
+    float mat[16];
+    float inv[16];
+    float size_x, size_y;
+} sortgpu_quad;
+
+/* Build options (Fbvh::Serialize, accel/fast_bvh.h:159-162). width 8 = OBVH, 4 = QBVH.
+ * Zero fields take the defaults: width 8, max_depth 28, max_leaf 16 (the Blender exporter's
+ * defaults); build_threads 0 = all host cores. */
+typedef struct sortgpu_build_opts {
+    uint32_t width;
+    uint32_t max_depth;
+    uint32_t max_leaf;
+    uint32_t build_threads;
+} sortgpu_build_opts;
+
+typedef struct sortgpu_ctx sortgpu_ctx;
+typedef struct sortgpu_accel sortgpu_accel;
+
+/* ---- context ---- */
+
+/* Binds the library to a HIP device. device < 0 creates a host-only context: builds run and
+ * can be inspected, but no device memory is used and queries return SORTGPU_ERR_STATE. */
+int sortgpu_ctx_create(int device, sortgpu_ctx** out);
+void sortgpu_ctx_destroy(sortgpu_ctx* ctx);
+/* Last error message of the context (or of the library when ctx is NULL). */
+const char* sortgpu_last_error(const sortgpu_ctx* ctx);
+
+/* ---- accelerator: Build (Accelerator::Build, accel/accelerator.h:128; Fbvh::Build,
+ *      accel/fast_bvh.hpp:109-136) ----
+ * pos: nverts x 3 floats; idx: ntris x 3 vertex indices; quads: nquads records. The SAH
+ * 8-wide (or 4-wide) tree is built on the host, flattened into the device layout and
+ * uploaded. Input arrays are copied; the caller keeps ownership. An empty scene yields a
+ * valid handle whose queries all miss (m_isValid = false, fast_bvh.hpp:112-114). */
+int sortgpu_accel_build(sortgpu_ctx* ctx, const float* pos, uint32_t nverts, const uint32_t* idx,
+                        uint32_t ntris, const sortgpu_quad* quads, uint32_t nquads,
+                        const sortgpu_build_opts* opts, sortgpu_accel** out);
+void sortgpu_accel_destroy(sortgpu_accel* accel);
+/* 1 when the tree holds at least one primitive (Accelerator::m_isValid). */
+int sortgpu_accel_valid(const sortgpu_accel* accel);
+/* Enlarged scene box used by the root test (core/scene.cpp:145-154): min xyz, max xyz. */
+int sortgpu_accel_bbox(const sortgpu_accel* accel, float out6[6]);
+/* out[0..7] = width, depth, inner nodes, leaves, primitives, max stack entries,
+ *             device bytes (nodes + primitive records), build milliseconds. */
+int sortgpu_accel_info(const sortgpu_accel* accel, uint64_t out[8]);
+/* Primitive order after the build (Fbvh::m_bvhpri), nprims entries. */
+int sortgpu_accel_prim_order(const sortgpu_accel* accel, uint32_t* out, uint64_t cap);
+/* Pre-order (child_cnt, pri_offset, pri_cnt) triples of the tree; returns the number of
+ * uint32 values (call with out = NULL to size), or a negative status. */
+int64_t sortgpu_accel_tree_dump(const sortgpu_accel* accel, uint32_t* out, uint64_t cap);
+
+/* ---- queries ---- */
+
+/* Closest hit for n rays (Accelerator::GetIntersect, accel/accelerator.h:74; Obvh traversal
+ * accel/fast_bvh.hpp:305-504, with Scene::GetIntersect's t = FLT_MAX reset,
+ * core/scene.cpp:182-185). d_rays: n records; d_hits: n records. */
+int sortgpu_trace_closest(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
+                          sortgpu_hit* d_hits, void* stream);
+
+/* Any hit / occlusion for n rays (Accelerator::IsOccluded, accel/accelerator.h:85; live form:
+ * GetIntersect with query_shadow = true, fast_bvh.hpp:362-373, accelerator.cpp:26-39).
+ * d_occluded[i] = 1 when some primitive is hit with tmin < t <= tmax, else 0. */
+int sortgpu_trace_any(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
+                      uint8_t* d_occluded, void* stream);
+
+/* Traversal work counters for n rays, synchronous: out[0..3] = inner nodes tested, leaves
+ * visited, triangles tested, quads tested (the reference's OBVH statistics,
+ * fast_bvh.hpp:74-97,407). mode 0 = closest, 1 = any-hit. Results are also written to
+ * d_hits / d_occluded exactly as by the plain entry points. */
+int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n, int mode,
+                           sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4],
+                           void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SORTGPU_H */

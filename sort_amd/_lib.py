@@ -1,0 +1,87 @@
+"""ctypes binding of libsortgpu.so (the C ABI declared in include/sortgpu.h).
+
+The library is loaded from this package directory only. A missing library is an error: there
+is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsortgpu.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "sortgpu.h")
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_OOM = -3
+ERR_BUDGET = -4
+ERR_STATE = -5
+
+
+class SortGpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sortgpu error {code}: {msg}")
+        self.code = code
+
+
+class Quad(ctypes.Structure):
+    _fields_ = [("mat", ctypes.c_float * 16), ("inv", ctypes.c_float * 16),
+                ("size_x", ctypes.c_float), ("size_y", ctypes.c_float)]
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("max_leaf", ctypes.c_uint32), ("build_threads", ctypes.c_uint32)]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_I = ctypes.c_int
+
+_SIGS = {
+    "sortgpu_ctx_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "sortgpu_ctx_destroy": (None, [_P]),
+    "sortgpu_last_error": (ctypes.c_char_p, [_P]),
+    "sortgpu_accel_build": (_I, [_P, _P, _U32, _P, _U32, _P, _U32, ctypes.POINTER(BuildOpts), ctypes.POINTER(_P)]),
+    "sortgpu_accel_destroy": (None, [_P]),
+    "sortgpu_accel_valid": (_I, [_P]),
+    "sortgpu_accel_bbox": (_I, [_P, _P]),
+    "sortgpu_accel_info": (_I, [_P, _P]),
+    "sortgpu_accel_prim_order": (_I, [_P, _P, ctypes.c_uint64]),
+    "sortgpu_accel_tree_dump": (ctypes.c_int64, [_P, _P, ctypes.c_uint64]),
+    "sortgpu_trace_closest": (_I, [_P, _P, _U32, _P, _P]),
+    "sortgpu_trace_any": (_I, [_P, _P, _U32, _P, _P]),
+    "sortgpu_trace_counters": (_I, [_P, _P, _U32, _I, _P, _P, _P, _P]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Every function the public header declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sortgpu_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(rc: int, ctx=None) -> int:
+    if rc < 0:
+        msg = lib().sortgpu_last_error(ctx)
+        raise SortGpuError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,293 @@
+// C ABI implementation: context, accelerator build/upload, ray queries.
+// Declarations and reference citations: include/sortgpu.h.
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "../../include/sortgpu.h"
+#include "api_internal.h"
+#include "bvh_build.h"
+#include "trace_launch.h"
+
+using namespace sortgpu;
+
+namespace {
+thread_local std::string g_lib_error;
+}
+
+namespace sortgpu {
+
+void set_error(sortgpu_ctx* ctx, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) {
+        std::lock_guard<std::mutex> l(ctx->mu);
+        ctx->err = buf;
+    }
+    g_lib_error = buf;
+}
+
+int hip_fail(sortgpu_ctx* ctx, hipError_t e, const char* what) {
+    set_error(ctx, "%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? SORTGPU_ERR_OOM : SORTGPU_ERR_HIP;
+}
+
+int ctx_bind(sortgpu_ctx* ctx) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipGetDevice");
+    if (cur != ctx->device) {
+        e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    }
+    return SORTGPU_OK;
+}
+
+}  // namespace sortgpu
+
+extern "C" {
+
+int sortgpu_ctx_create(int device, sortgpu_ctx** out) {
+    if (!out) {
+        set_error(nullptr, "sortgpu_ctx_create: out is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    *out = nullptr;
+    sortgpu_ctx* ctx = new (std::nothrow) sortgpu_ctx();
+    if (!ctx) return SORTGPU_ERR_OOM;
+    ctx->device = device;
+    ctx->host_only = device < 0;
+    if (!ctx->host_only) {
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || device >= count) {
+            set_error(nullptr, "sortgpu_ctx_create: device %d not available (%s)", device,
+                      e != hipSuccess ? hipGetErrorString(e) : "index out of range");
+            delete ctx;
+            return SORTGPU_ERR_HIP;
+        }
+        e = hipSetDevice(device);
+        if (e != hipSuccess) {
+            int rc = hip_fail(nullptr, e, "hipSetDevice");
+            delete ctx;
+            return rc;
+        }
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, device);
+        if (e != hipSuccess) {
+            int rc = hip_fail(nullptr, e, "hipGetDeviceProperties");
+            delete ctx;
+            return rc;
+        }
+        ctx->num_cus = prop.multiProcessorCount;
+        ctx->hbm_bytes = prop.totalGlobalMem;
+        std::snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
+    }
+    *out = ctx;
+    return SORTGPU_OK;
+}
+
+void sortgpu_ctx_destroy(sortgpu_ctx* ctx) { delete ctx; }
+
+const char* sortgpu_last_error(const sortgpu_ctx* ctx) {
+    if (ctx) return ctx->err.c_str();
+    return g_lib_error.c_str();
+}
+
+int sortgpu_accel_build(sortgpu_ctx* ctx, const float* pos, uint32_t nverts, const uint32_t* idx,
+                        uint32_t ntris, const sortgpu_quad* quads, uint32_t nquads,
+                        const sortgpu_build_opts* opts, sortgpu_accel** out) {
+    if (!ctx || !out) {
+        set_error(ctx, "sortgpu_accel_build: ctx and out must be non-NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    *out = nullptr;
+    BuildInput in;
+    in.pos = pos;
+    in.nverts = nverts;
+    in.idx = idx;
+    in.ntris = ntris;
+    static_assert(sizeof(sortgpu_quad) == sizeof(QuadIn), "quad record mismatch");
+    in.quads = reinterpret_cast<const QuadIn*>(quads);
+    in.nquads = nquads;
+    if (opts) {
+        if (opts->width) in.width = opts->width;
+        if (opts->max_depth) in.max_depth = opts->max_depth;
+        if (opts->max_leaf) in.max_leaf = opts->max_leaf;
+        in.threads = opts->build_threads;
+    }
+    sortgpu_accel* acc = new (std::nothrow) sortgpu_accel();
+    if (!acc) return SORTGPU_ERR_OOM;
+    acc->ctx = ctx;
+    std::string err;
+    try {
+        if (!build_tree(in, acc->host, err)) {
+            set_error(ctx, "sortgpu_accel_build: %s", err.c_str());
+            delete acc;
+            return SORTGPU_ERR_ARG;
+        }
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "sortgpu_accel_build: host allocation failed");
+        delete acc;
+        return SORTGPU_ERR_OOM;
+    } catch (...) {
+        set_error(ctx, "sortgpu_accel_build: unexpected failure");
+        delete acc;
+        return SORTGPU_ERR_ARG;
+    }
+    if (acc->host.max_stack > trace_max_stack_supported()) {
+        set_error(ctx, "sortgpu_accel_build: tree depth %u needs %u stack entries (limit %u)", acc->host.depth,
+                  acc->host.max_stack, trace_max_stack_supported());
+        delete acc;
+        return SORTGPU_ERR_ARG;
+    }
+    TreeView& v = acc->view;
+    std::memset(&v, 0, sizeof(v));
+    v.root = acc->host.root_ref;
+    v.valid = acc->host.valid ? 1u : 0u;
+    v.max_stack = acc->host.max_stack;
+    v.width = acc->host.width;
+    std::memcpy(v.bbox, acc->host.bbox, sizeof(v.bbox));
+    if (!ctx->host_only) {
+        int rc = upload_accel(acc);
+        if (rc != SORTGPU_OK) {
+            delete acc;
+            return rc;
+        }
+    }
+    *out = acc;
+    return SORTGPU_OK;
+}
+
+void sortgpu_accel_destroy(sortgpu_accel* acc) {
+    if (!acc) return;
+    free_accel_device(acc);
+    delete acc;
+}
+
+int sortgpu_accel_valid(const sortgpu_accel* acc) { return acc && acc->host.valid ? 1 : 0; }
+
+int sortgpu_accel_bbox(const sortgpu_accel* acc, float out6[6]) {
+    if (!acc || !out6) return SORTGPU_ERR_ARG;
+    std::memcpy(out6, acc->host.bbox, 6 * sizeof(float));
+    return SORTGPU_OK;
+}
+
+int sortgpu_accel_info(const sortgpu_accel* acc, uint64_t out[8]) {
+    if (!acc || !out) return SORTGPU_ERR_ARG;
+    const BuiltTree& h = acc->host;
+    out[0] = h.width;
+    out[1] = h.depth;
+    out[2] = h.n_inner;
+    out[3] = h.n_leaves;
+    out[4] = h.nprims;
+    out[5] = h.max_stack;
+    out[6] = h.node_bytes.size() + h.prims.size() * sizeof(PrimRec) + h.quads.size() * sizeof(QuadRec);
+    out[7] = (uint64_t)(h.build_ms + 0.5);
+    return SORTGPU_OK;
+}
+
+int sortgpu_accel_prim_order(const sortgpu_accel* acc, uint32_t* out, uint64_t cap) {
+    if (!acc || (!out && cap)) return SORTGPU_ERR_ARG;
+    const auto& po = acc->host.prim_order;
+    const uint64_t n = po.size() < cap ? po.size() : cap;
+    if (n) std::memcpy(out, po.data(), n * sizeof(uint32_t));
+    return SORTGPU_OK;
+}
+
+int64_t sortgpu_accel_tree_dump(const sortgpu_accel* acc, uint32_t* out, uint64_t cap) {
+    if (!acc) return SORTGPU_ERR_ARG;
+    const auto& d = acc->host.dump;
+    if (out) {
+        const uint64_t n = d.size() < cap ? d.size() : cap;
+        if (n) std::memcpy(out, d.data(), n * sizeof(uint32_t));
+    }
+    return (int64_t)d.size();
+}
+
+static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, void* d_out, bool any,
+                        bool stats, uint64_t* out4, void* stream) {
+    if (!acc) {
+        set_error(nullptr, "trace: accel is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    sortgpu_ctx* ctx = acc->ctx;
+    if (ctx->host_only) {
+        set_error(ctx, "trace: host-only context has no device");
+        return SORTGPU_ERR_STATE;
+    }
+    if (n == 0) {
+        if (out4) std::memset(out4, 0, 4 * sizeof(uint64_t));
+        return SORTGPU_OK;
+    }
+    if (!d_rays || !d_out) {
+        set_error(ctx, "trace: null ray or output pointer");
+        return SORTGPU_ERR_ARG;
+    }
+    int rc = ctx_bind(ctx);
+    if (rc != SORTGPU_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    TraceArgs a;
+    a.tree = acc->view;
+    a.rays = d_rays;
+    a.n = n;
+    a.out = d_out;
+    const uint32_t slot = acc->slot.fetch_add(1) % kCounterSlots;
+    a.counter = acc->d_counters + slot * kCounterStride;
+    a.error = acc->d_error;
+    a.stats = nullptr;
+    if (stats) {
+        hipError_t e = hipMemsetAsync(acc->d_stats, 0, 4 * sizeof(unsigned long long), s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(stats)");
+        a.stats = acc->d_stats;
+    }
+    hipError_t e = launch_trace(a, any, stats, s, ctx->num_cus);
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    if (stats) {
+        unsigned long long h[4];
+        e = hipMemcpyAsync(h, acc->d_stats, sizeof(h), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(stats)");
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+        for (int k = 0; k < 4; k++) out4[k] = h[k];
+        uint32_t err = 0;
+        e = hipMemcpy(&err, acc->d_error, sizeof(err), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(error)");
+        if (err) {
+            set_error(ctx, "trace: traversal stack overflow (flags %u)", err);
+            return SORTGPU_ERR_STATE;
+        }
+    }
+    return SORTGPU_OK;
+}
+
+int sortgpu_trace_closest(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, sortgpu_hit* d_hits,
+                          void* stream) {
+    return trace_common(acc, d_rays, n, d_hits, false, false, nullptr, stream);
+}
+
+int sortgpu_trace_any(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, uint8_t* d_occluded,
+                      void* stream) {
+    return trace_common(acc, d_rays, n, d_occluded, true, false, nullptr, stream);
+}
+
+int sortgpu_trace_counters(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, int mode,
+                           sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4], void* stream) {
+    if (!out || (mode != 0 && mode != 1)) {
+        set_error(acc ? acc->ctx : nullptr, "trace_counters: bad mode or null out");
+        return SORTGPU_ERR_ARG;
+    }
+    void* d_out = mode == 0 ? static_cast<void*>(d_hits) : static_cast<void*>(d_occluded);
+    return trace_common(acc, d_rays, n, d_out, mode == 1, true, out, stream);
+}
+
+}  // extern "C"

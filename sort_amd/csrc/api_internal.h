@@ -1,0 +1,50 @@
+// Internal definitions behind the opaque C handles.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+
+#include "../../include/sortgpu.h"
+#include "bvh_build.h"
+#include "layout.h"
+
+struct sortgpu_ctx {
+    int device = -1;
+    bool host_only = true;
+    int num_cus = 0;
+    size_t hbm_bytes = 0;
+    char arch[64] = {0};
+    std::mutex mu;
+    std::string err;
+};
+
+namespace sortgpu {
+// Work-counter ring: each query takes the next slot so queries on different streams that
+// share one accelerator never share a counter (unless more than kCounterSlots are in flight).
+constexpr uint32_t kCounterSlots = 256;
+constexpr uint32_t kCounterStride = 16;  // 64 B apart
+}  // namespace sortgpu
+
+struct sortgpu_accel {
+    sortgpu_ctx* ctx = nullptr;
+    sortgpu::BuiltTree host;
+    sortgpu::TreeView view;
+    void* d_nodes = nullptr;
+    sortgpu::PrimRec* d_prims = nullptr;
+    sortgpu::QuadRec* d_quads = nullptr;
+    uint32_t* d_counters = nullptr;
+    uint32_t* d_error = nullptr;
+    unsigned long long* d_stats = nullptr;
+    std::atomic<uint32_t> slot{0};
+    size_t device_bytes = 0;
+};
+
+namespace sortgpu {
+void set_error(sortgpu_ctx* ctx, const char* fmt, ...);
+int hip_fail(sortgpu_ctx* ctx, hipError_t e, const char* what);
+int ctx_bind(sortgpu_ctx* ctx);
+int upload_accel(sortgpu_accel* acc);
+void free_accel_device(sortgpu_accel* acc);
+}  // namespace sortgpu

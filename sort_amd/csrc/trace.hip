@@ -1,0 +1,245 @@
+// Closest-hit and any-hit traversal kernels over a flattened Fast-BVH (8- or 4-wide).
+//
+// Execution model: persistent waves, one ray per lane. A lane whose stack empties writes its
+// result and, together with the other idle lanes of its wave, takes the next rays from a
+// device work counter (one atomic per refill, taken when at least kRefill lanes are idle or the
+// whole wave is), so long rays do not hold a wave's other 63 lanes idle.
+//
+// Traversal stack: entries (child reference, entry distance). The top kStackLds entries of
+// every lane live in LDS ([entry][thread] so a wave's accesses to one entry hit 64 consecutive
+// dwords); deeper entries spill to a private per-lane array in scratch. The host picks the
+// spill depth from the tree's exact worst case (W-1)*(depth-1)+1, so no push can go past it;
+// the kernel still bounds-checks every push and raises an error flag instead of writing.
+//
+// The walk is SORT's Fbvh::GetIntersect (accel/fast_bvh.hpp:305-504): pop, prune when the
+// current closest t is below the entry distance, test the leaf's primitives in order, or test
+// the W child boxes and push the hit children farthest first (ties: lower lane pushed first).
+#include <hip/hip_runtime.h>
+
+#include "traverse.h"
+#include "trace_launch.h"
+
+#pragma clang fp contract(off)
+
+namespace sortgpu {
+
+constexpr int kBlock = 256;
+constexpr int kStackLds = 16;
+constexpr int kRefill = 16;
+
+template <int W, bool ANY, bool STATS, int SPILL>
+__global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
+    __shared__ uint32_t s_ref[kStackLds * kBlock];
+    __shared__ float s_tm[kStackLds * kBlock];
+    constexpr int CAP = kStackLds + SPILL;
+    uint32_t sp_ref[SPILL > 0 ? SPILL : 1];
+    float sp_tm[SPILL > 0 ? SPILL : 1];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const NodeRec<W>* __restrict__ nodes = static_cast<const NodeRec<W>*>(a.tree.nodes);
+    const PrimRec* __restrict__ prims = a.tree.prims;
+
+    RayK r;
+    uint32_t ray_id = 0xffffffffu;
+    bool more = true;
+    int sp = 0;
+    float bt = FLT_MAX, bu = 0.f, bv = 0.f;
+    uint32_t bp = 0xffffffffu;
+    uint32_t n_nodes = 0, n_leaves = 0, n_tris = 0, n_quads = 0;
+    bool overflow = false;
+
+    auto push = [&](int e, uint32_t ref, float t) {
+        if (e < kStackLds) {
+            s_ref[e * kBlock + tid] = ref;
+            s_tm[e * kBlock + tid] = t;
+        } else if (e < CAP) {
+            sp_ref[e - kStackLds] = ref;
+            sp_tm[e - kStackLds] = t;
+        } else {
+            overflow = true;
+        }
+    };
+
+    while (true) {
+        const bool idle = sp == 0;
+        const uint64_t want = __ballot(idle && more);
+        const uint64_t busy = __ballot(!idle);
+        if (want != 0 && (busy == 0 || __popcll(want) >= kRefill)) {
+            const uint32_t cnt = (uint32_t)__popcll(want);
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(a.counter, cnt);
+            base = __shfl(base, leader);
+            if (idle && more) {
+                const uint32_t my = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+                if (my < a.n) {
+                    ray_id = my;
+                    const float4* rp = reinterpret_cast<const float4*>(a.rays) + 2 * (size_t)my;
+                    const float4 q0 = rp[0], q1 = rp[1];
+                    r.ox = q0.x; r.oy = q0.y; r.oz = q0.z; r.tmin = q0.w;
+                    r.dx = q1.x; r.dy = q1.y; r.dz = q1.z; r.tmax = q1.w;
+                    ray_setup(r);
+                    bt = FLT_MAX;
+                    bp = 0xffffffffu;
+                    bu = bv = 0.f;
+                    const float rmin = a.tree.valid ? root_slab(r, a.tree.bbox) : -1.0f;
+                    if (rmin >= 0.0f) {
+                        push(0, a.tree.root, rmin);
+                        sp = 1;
+                    } else {
+                        if (ANY) static_cast<uint8_t*>(a.out)[my] = 0;
+                        else reinterpret_cast<float4*>(a.out)[my] =
+                                 make_float4(FLT_MAX, __uint_as_float(0xffffffffu), 0.f, 0.f);
+                        ray_id = 0xffffffffu;
+                    }
+                } else {
+                    more = false;
+                }
+            }
+        }
+        if (__ballot(sp > 0) == 0) {
+            if (__ballot(more) == 0) break;
+            continue;
+        }
+        if (sp > 0) {
+            --sp;
+            uint32_t ref;
+            float tm;
+            if (sp < kStackLds) {
+                ref = s_ref[sp * kBlock + tid];
+                tm = s_tm[sp * kBlock + tid];
+            } else {
+                const int e = sp - kStackLds < SPILL ? sp - kStackLds : 0;
+                ref = sp_ref[e];
+                tm = sp_tm[e];
+            }
+            bool occluded = false;
+            if (!(bt < tm)) {
+                if (ref & REF_LEAF) {
+                    if (STATS) n_leaves++;
+                    uint32_t i = ref & ~REF_LEAF;
+                    while (true) {
+                        const PrimRec pr = load_prim(prims, i);
+                        float t, u, v;
+                        bool hit = false;
+                        if (pr.flags & PR_QUAD) {
+                            if (STATS) n_quads++;
+                            if (quad_test(r, a.tree.quads[pr.aux], bt, t)) {
+                                bt = t;
+                                bp = pr.prim;
+                                bu = bv = 0.f;
+                                hit = true;
+                            }
+                        } else {
+                            if (STATS) n_tris++;
+                            if (tri_test(r, pr, t, u, v) && t < bt) {
+                                bt = t;
+                                bp = pr.prim;
+                                bu = u;
+                                bv = v;
+                                hit = true;
+                            }
+                        }
+                        if (ANY && hit) {
+                            occluded = true;
+                            break;
+                        }
+                        if (pr.flags & PR_LAST) break;
+                        ++i;
+                    }
+                } else {
+                    if (STATS) n_nodes++;
+                    float f[W];
+                    uint32_t ch[W];
+                    const uint32_t m = node_test<W>(r, nodes, ref, f, ch);
+                    const int nh = __popc(m);
+                    uint32_t elig = m;
+                    if (nh >= 3) {
+                        // the >=3 branch pushes by repeated argmax over f > -1 (fast_bvh.hpp:438-453)
+#pragma unroll
+                        for (int j = 0; j < W; j++)
+                            if (!(f[j] > -1.0f)) elig &= ~(1u << j);
+                    }
+#pragma unroll
+                    for (int j = 0; j < W; j++) {
+                        if (elig & (1u << j)) {
+                            int rank = 0;
+#pragma unroll
+                            for (int k = 0; k < W; k++) {
+                                if (k == j) continue;
+                                const bool before = (f[k] > f[j]) | ((f[k] == f[j]) & (k < j));
+                                rank += ((elig >> k) & 1u) & (before ? 1u : 0u);
+                            }
+                            push(sp + rank, ch[j], f[j]);
+                        }
+                    }
+                    sp += __popc(elig);
+                    // Only reachable on overflow (the host sizes the stack for the tree): every
+                    // slot below CAP was written by this push, so popping stays defined.
+                    if (sp > CAP) sp = CAP;
+                }
+            }
+            if (occluded) sp = 0;
+            if (sp == 0) {
+                if (ANY) static_cast<uint8_t*>(a.out)[ray_id] = occluded ? 1 : 0;
+                else reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);
+                ray_id = 0xffffffffu;
+            }
+        }
+    }
+
+    if (overflow) atomicOr(a.error, 1u);
+    if (STATS) {
+        unsigned long long v[4] = {n_nodes, n_leaves, n_tris, n_quads};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            unsigned long long x = v[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+            if (lane == 0 && x) atomicAdd(a.stats + k, x);
+        }
+    }
+}
+
+__global__ void k_reset(uint32_t* counter) {
+    if (threadIdx.x == 0) *counter = 0;
+}
+
+template <int W, bool ANY, bool STATS, int SPILL>
+static hipError_t launch_one(const TraceArgs& a, hipStream_t stream, int num_cus) {
+    auto kern = k_trace<W, ANY, STATS, SPILL>;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    const uint64_t want = ((uint64_t)a.n + kBlock - 1) / kBlock;
+    uint64_t grid = (uint64_t)num_cus * (uint64_t)per_cu;
+    if (want < grid) grid = want;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, a.counter);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kBlock), 0, stream, a);
+    return hipGetLastError();
+}
+
+template <int W, bool ANY, bool STATS>
+static hipError_t launch_w(const TraceArgs& a, hipStream_t stream, int num_cus) {
+    const uint32_t need = a.tree.max_stack;
+    if (need <= kStackLds) return launch_one<W, ANY, STATS, 0>(a, stream, num_cus);
+    if (need <= kStackLds + 48) return launch_one<W, ANY, STATS, 48>(a, stream, num_cus);
+    if (need <= kStackLds + 240) return launch_one<W, ANY, STATS, 240>(a, stream, num_cus);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t stream, int num_cus) {
+    if (a.n == 0) return hipSuccess;
+    if (a.tree.width == 8) {
+        if (any) return stats ? launch_w<8, true, true>(a, stream, num_cus) : launch_w<8, true, false>(a, stream, num_cus);
+        return stats ? launch_w<8, false, true>(a, stream, num_cus) : launch_w<8, false, false>(a, stream, num_cus);
+    }
+    if (any) return stats ? launch_w<4, true, true>(a, stream, num_cus) : launch_w<4, true, false>(a, stream, num_cus);
+    return stats ? launch_w<4, false, true>(a, stream, num_cus) : launch_w<4, false, false>(a, stream, num_cus);
+}
+
+uint32_t trace_max_stack_supported() { return kStackLds + 240; }
+
+}  // namespace sortgpu

@@ -1,0 +1,25 @@
+// Host-visible launch interface of the traversal kernels.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+namespace sortgpu {
+
+struct TraceArgs {
+    TreeView tree;
+    const void* rays;              // n x 32 B ray records (device)
+    uint32_t n;
+    void* out;                     // closest: n x 16 B hits; any: n x u8 (device)
+    uint32_t* counter;             // work counter (device), reset by the launch
+    uint32_t* error;               // error flags (device): bit0 = stack overflow
+    unsigned long long* stats;     // 4 counters (device), only with stats
+};
+
+// Launches the closest (any = false) or any-hit traversal on `stream`.
+hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t stream, int num_cus);
+// Deepest stack (entries) the kernels can hold.
+uint32_t trace_max_stack_supported();
+
+}  // namespace sortgpu
