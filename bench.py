@@ -1,0 +1,183 @@
+"""Benchmark: incoherent closest-hit Mrays/s on the 1M-triangle intersection workload.
+
+Workload (one GPU): a seeded 2^20-triangle soup (centroids U[0,1]^3, sizes LogNormal(ln 0.004,
+0.5)), its SAH OBVH (max depth 28, max leaf 16) resident in HBM, and three 4,194,304-ray sets:
+coherent 2048^2 pinhole rays, incoherent cosine-hemisphere bounce rays from random camera hits
+(tmin 1e-4) and shadow segments to a 0.25^2 light quad (tmin 0.01, tmax len-0.01).
+
+A step is one closest-hit traversal of the 4M incoherent rays (rays and BVH already resident).
+value = incoherent Mrays/s over the timed steps. Coherent closest-hit and shadow any-hit rates
+are timed the same way and reported beside it.
+
+roofline: HBM-bound. Algorithmic bytes per ray = 32 (ray) + 16 (hit) + 224 * inner nodes
+visited + 36 * triangles tested, with the visit counts from the traversal's own counters (equal
+to the oracle's on these rays); achieved = bytes per launch / mean launch time (HIP events on
+the launch stream).
+
+cpu_baseline: the oracle (C++ AVX restatement of SORT's Obvh traversal, std::thread pool, 4096-ray
+chunks) on the same incoherent rays, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--width", type=int, default=8, choices=[4, 8])
+    p.add_argument("--cpu-runs", type=int, default=5)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--tris", type=int, default=1 << 20)
+    return p.parse_args()
+
+
+def timed(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        fn()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return wall, float(np.mean(per)), float(np.median(per))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
+                                   "(rays shard across GPUs with a replicated BVH, see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+
+    from sort_amd import Context, Obvh, Qbvh, scenes
+
+    t_setup = time.perf_counter()
+    pos, idx = scenes.soup(a.tris)
+    ctx = Context(0)
+    acc = (Obvh if a.width == 8 else Qbvh)(ctx).build(pos, idx)
+    info = acc.info()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+
+    cam = scenes.pinhole_rays(2048, 2048)
+    d_cam = torch.from_numpy(cam).to(dev)
+    cam_hits = acc.get_intersect(d_cam).cpu().numpy()
+    inc = scenes.incoherent_rays(cam, cam_hits, pos, idx)
+    sh = scenes.shadow_rays(cam, cam_hits, pos, idx)
+    d_inc = torch.from_numpy(inc).to(dev)
+    d_sh = torch.from_numpy(sh).to(dev)
+    n = d_inc.shape[0]
+    hits = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    occ = torch.empty((n,), dtype=torch.uint8, device=dev)
+    from sort_amd._lib import check, lib as _l
+    lib = _l()
+    s_ptr = stream.cuda_stream
+
+    def trace_inc():
+        check(lib.sortgpu_trace_closest(acc.handle, d_inc.data_ptr(), n, hits.data_ptr(), s_ptr))
+
+    def trace_cam():
+        check(lib.sortgpu_trace_closest(acc.handle, d_cam.data_ptr(), n, hits.data_ptr(), s_ptr))
+
+    def trace_sh():
+        check(lib.sortgpu_trace_any(acc.handle, d_sh.data_ptr(), n, occ.data_ptr(), s_ptr))
+
+    # visit counts for the bytes model (same traversal, counters on)
+    _, st_inc = acc.counters(d_inc, 0)
+    _, st_cam = acc.counters(d_cam, 0)
+    _, st_sh = acc.counters(d_sh, 1)
+    setup_s = time.perf_counter() - t_setup
+
+    node_b = 224 if a.width == 8 else 112
+
+    def bytes_per_ray(st, out_b):
+        return 32 + out_b + node_b * st["inner_nodes"] / n + 36 * (st["tris"] + st["quads"]) / n
+
+    # timed region: K steps of the incoherent closest-hit traversal
+    torch.cuda.synchronize()
+    wall, mean_ms, med_ms = timed(trace_inc, a.steps, a.warmup, stream)
+    inc_mrays = n * a.steps / wall / 1e6
+    _, cam_ms, _ = timed(trace_cam, max(5, a.steps // 5), 2, stream)
+    _, sh_ms, _ = timed(trace_sh, max(5, a.steps // 5), 2, stream)
+
+    b_inc = bytes_per_ray(st_inc, 16)
+    achieved = b_inc * n / (mean_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle.oracle import OracleBvh
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        ob = OracleBvh(pos, idx, width=a.width)
+        ob.closest(inc[: 1 << 16], threads=threads)  # warm-up
+        secs = []
+        for _ in range(a.cpu_runs):
+            st = {}
+            ob.closest(inc, threads=threads, stats=st)
+            secs.append(st["seconds"])
+        cpu_mrays = n / float(np.median(secs)) / 1e6
+        cpu = {"value": round(cpu_mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"all {n} incoherent rays, median of {a.cpu_runs} runs after a warm-up; oracle = "
+                         f"C++ AVX (-mavx -mno-fma) restatement of SORT's Obvh traversal, {threads} threads"}
+
+    out = {
+        "metric": "incoherent Mrays/s",
+        "value": round(inc_mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"1M-triangle seeded soup ({a.tris} tris), "
+                               f"{'OBVH' if a.width == 8 else 'QBVH'} depth 28 leaf 16, "
+                               "4,194,304 incoherent closest-hit rays per step",
+                   "rays_per_step": n, "width": a.width,
+                   "coherent_mrays_s": round(n / (cam_ms * 1e-3) / 1e6, 3),
+                   "shadow_mrays_s": round(n / (sh_ms * 1e-3) / 1e6, 3),
+                   "bvh_build_ms": info["build_ms"], "bvh_depth": info["depth"],
+                   "setup_s": round(setup_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_ray": round(b_inc, 1), "launch_ms_mean": round(mean_ms, 4),
+                     "launch_ms_median": round(med_ms, 4),
+                     "visits_per_ray": {"inner_nodes": round(st_inc["inner_nodes"] / n, 3),
+                                        "tris": round(st_inc["tris"] / n, 3)},
+                     "coherent": {"bytes_per_ray": round(bytes_per_ray(st_cam, 16), 1),
+                                  "frac": round(bytes_per_ray(st_cam, 16) * n / (cam_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "shadow": {"bytes_per_ray": round(bytes_per_ray(st_sh, 4), 1),
+                                "frac": round(bytes_per_ray(st_sh, 4) * n / (sh_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["config"]["gpu_over_cpu"] = round(inc_mrays / cpu["value"], 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
