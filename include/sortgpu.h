@@ -123,6 +123,68 @@ int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint
                            sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4],
                            void* stream);
 
+/* ---- path tracing (PathTracing::Li, integrator/pathtracing.cpp:37-258) ---- */
+
+#define SORTGPU_MAT_LAMBERT 0    /* Lambert (scatteringevent/bsdf/lambert.cpp)                    */
+#define SORTGPU_MAT_GGX 1        /* MicroFacetReflection, GGX, conductor Fresnel (microfacet.cpp)  */
+#define SORTGPU_MAT_DIELECTRIC 2 /* rough dielectric, exterior IOR 1 (dielectric.cpp)              */
+#define SORTGPU_MAT_DISNEY 3     /* Disney BRDF subset: no transmission / thin / SSS (disney.cpp)  */
+
+/* One BXDF per material (the configurations' materials; SORT builds them from TSL closures). */
+typedef struct sortgpu_material {
+    uint32_t type;
+    float base[3];     /* Lambert R, GGX R, dielectric reflectance, Disney base color */
+    float trans[3];    /* dielectric transmittance */
+    float roughness;   /* GGX / dielectric / Disney */
+    float ior;         /* dielectric interior IOR */
+    float eta[3];      /* conductor Fresnel (GGX) */
+    float k[3];
+    float metallic, specular, specular_tint, sheen, sheen_tint, clearcoat, clearcoat_gloss;
+} sortgpu_material;
+
+/* Shading data over an accelerator's geometry. Every quad of the accelerator is a one-sided
+ * area light (AreaLight + Quad, light/area.cpp) with the default white Lambert surface
+ * (material/matmanager.h:54-57). */
+typedef struct sortgpu_scene_desc {
+    const float* normals;          /* nverts x 3 shading normals, or NULL: normalize(cross(p1-p0, p2-p0)) */
+    const float* tangents;         /* nverts x 3, or NULL: normalize(p1 - p0) per triangle */
+    const uint32_t* tri_material;  /* ntris material indices, or NULL: material 0 */
+    const sortgpu_material* materials;
+    uint32_t nmaterials;
+    const float* quad_radiance;    /* nquads x 3 emitted radiance */
+    float env_radiance[3];         /* constant environment (AmbientLight); all 0 = none */
+} sortgpu_scene_desc;
+
+/* Pinhole perspective camera (PerspectiveCamera, camera/perspective.cpp:26-104). */
+typedef struct sortgpu_camera {
+    float eye[3];
+    float target[3];
+    float up[3];
+    float fov_y;   /* vertical field of view, radians */
+} sortgpu_camera;
+
+typedef struct sortgpu_scene sortgpu_scene;
+
+/* Uploads shading data for `accel` (which must outlive the scene). */
+int sortgpu_scene_create(sortgpu_accel* accel, const sortgpu_scene_desc* desc, sortgpu_scene** out);
+void sortgpu_scene_destroy(sortgpu_scene* scene);
+
+/* Renders width x height at spp samples per pixel, max_depth bounces (Integrator
+ * max_recursive_depth, integrator.h:127), with Russian roulette after the fourth bounce. Writes
+ * the per-pixel mean of the valid samples (image_evaluation.cpp:222-244) to d_film_rgb
+ * (width * height * 3 floats, device). Rendering of one scene object is serial. */
+int sortgpu_render(sortgpu_scene* scene, const sortgpu_camera* camera, uint32_t width, uint32_t height,
+                   uint32_t spp, uint32_t max_depth, uint64_t seed, float* d_film_rgb, void* stream);
+
+/* Batched Integrator::Li (integrator.h:56): radiance along n caller rays. d_keys holds per ray
+ * (pixel, sample) — the random-number key of the path. d_rgb: n x 3 floats (device). */
+int sortgpu_li(sortgpu_scene* scene, const sortgpu_ray* d_rays, const uint32_t* d_keys, uint32_t n,
+               uint32_t max_depth, uint64_t seed, float* d_rgb, void* stream);
+
+/* Totals of the last render / li call (synchronous): out[0] extension rays, out[1] shadow
+ * rays, out[2] paths, out[3] microseconds of the last call (host wall clock to completion). */
+int sortgpu_render_stats(sortgpu_scene* scene, uint64_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif

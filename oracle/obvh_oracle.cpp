@@ -960,6 +960,24 @@ double oracle_trace(OracleHandle* h, const void* rays_, uint64_t n, void* hits_,
     return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// One ray on the calling thread with a per-thread stack (the path tracer's per-vertex queries,
+// like a render thread calling Scene::GetIntersect). mode 0 closest (hit16), 1 any (returns 1
+// when occluded).
+int oracle_ray1(OracleHandle* h, const void* ray_, void* hit_, int mode) {
+    const RayRec* r = static_cast<const RayRec*>(ray_);
+    Hit hh;
+    bool res;
+    if (h->width == 8) {
+        thread_local std::vector<std::pair<const Node<8>*, float>> st;
+        res = B<8>(h)->GetIntersect(toRay(*r), hh, mode == 1, st, nullptr);
+    } else {
+        thread_local std::vector<std::pair<const Node<4>*, float>> st;
+        res = B<4>(h)->GetIntersect(toRay(*r), hh, mode == 1, st, nullptr);
+    }
+    if (mode == 0) *static_cast<Hit*>(hit_) = hh;
+    return res ? 1 : 0;
+}
+
 // Brute force over all primitives (same per-triangle arithmetic). mode as above.
 void oracle_brute(OracleHandle* h, const void* rays_, uint64_t n, void* hits_, uint8_t* occ, int mode) {
     const RayRec* rays = static_cast<const RayRec*>(rays_);

@@ -212,3 +212,123 @@ def simd_minreduce8(x):
 def simd_movemask8(x):
     a = np.ascontiguousarray(np.asarray(x, np.float32).reshape(8))
     return int(lib().oracle_simd_movemask8(a.ctypes.data))
+
+
+# ---- path tracer (pt_oracle.cpp) ----------------------------------------------------------------
+
+def _pt_sigs():
+    l = lib()
+    if getattr(l, "_pt_ready", False):
+        return l
+    l.oracle_pt_create.restype = _P
+    l.oracle_pt_create.argtypes = [_P, _P, ctypes.c_uint32, _P, ctypes.c_uint32, _P, _P, _P, _P, ctypes.c_uint32,
+                                   _P, ctypes.c_uint32, _P, _P, _P]
+    l.oracle_pt_destroy.argtypes = [_P]
+    l.oracle_pt_render.restype = ctypes.c_double
+    l.oracle_pt_render.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _P, _P]
+    l.oracle_pt_li.argtypes = [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _P]
+    l.oracle_bxdf.argtypes = [_P, ctypes.c_int, ctypes.c_uint32, _P, _P, _P, _P]
+    l._pt_ready = True
+    return l
+
+
+MAT_FIELDS = ["metallic", "specular", "specular_tint", "sheen", "sheen_tint", "clearcoat", "clearcoat_gloss"]
+
+
+def material_record(m) -> np.ndarray:
+    """22-word record: type, base[3], trans[3], roughness, ior, eta[3], k[3], 7 Disney scalars."""
+    rec = np.zeros(22, np.float32)
+    rec[1:4] = m["base"]
+    rec[4:7] = m["trans"]
+    rec[7] = m["roughness"]
+    rec[8] = m["ior"]
+    rec[9:12] = m["eta"]
+    rec[12:15] = m["k"]
+    rec[15:22] = [m[f] for f in MAT_FIELDS]
+    rec.view(np.uint32)[0] = m["type"]
+    return rec
+
+
+class OraclePT:
+    """The restated SORT path tracer over an OracleBvh (same geometry, quads first-class)."""
+
+    def __init__(self, scene, width=8, max_depth=28, max_leaf=16):
+        l = _pt_sigs()
+        self.bvh = OracleBvh(scene["pos"], scene["idx"], scene["quads"], width=width, max_depth=max_depth,
+                             max_leaf=max_leaf)
+        self._keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            x = np.ascontiguousarray(np.asarray(a, dt))
+            self._keep.append(x)
+            return x
+
+        pos = arr(scene["pos"], np.float32)
+        idx = arr(scene["idx"], np.uint32)
+        nrm = arr(scene.get("normals"), np.float32)
+        tan = arr(scene.get("tangents"), np.float32)
+        tm = arr(scene.get("tri_material"), np.uint32)
+        mats = arr(np.stack([material_record(m) for m in scene["materials"]]), np.float32)
+        q34 = arr(quads_array(scene["quads"]), np.float32)
+        qr = arr(scene.get("quad_radiance", np.zeros((len(scene["quads"]), 3))), np.float32)
+        env = arr(scene.get("env", (0, 0, 0)), np.float32)
+        bb = arr(self.bvh.bbox(), np.float32)
+        self.h = l.oracle_pt_create(self.bvh.h, _ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(nrm), _ptr(tan),
+                                    _ptr(tm), _ptr(mats), len(mats), _ptr(q34), len(q34), _ptr(qr), _ptr(env),
+                                    _ptr(bb))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _pt_sigs().oracle_pt_destroy(self.h)
+            self.h = None
+
+    def render(self, cam, width, height, spp, max_depth, seed, threads=8, crop=None, stats=None):
+        """cam: dict(eye, target, up, fov_y_deg). crop = (pix0, pix1) linear pixel range.
+        Returns (npix, 3) radiance (mean of valid samples)."""
+        import math
+        c = np.array([*cam["eye"], *cam["target"], *cam["up"], math.radians(cam["fov_y_deg"])], np.float32)
+        p0, p1 = crop if crop else (0, width * height)
+        out = np.zeros((p1 - p0, 3), np.float32)
+        st = np.zeros(2, np.uint64)
+        secs = _pt_sigs().oracle_pt_render(self.h, c.ctypes.data, width, height, p0, p1, spp, max_depth, seed,
+                                           threads, out.ctypes.data, st.ctypes.data)
+        if stats is not None:
+            stats.update(extension_rays=int(st[0]), shadow_rays=int(st[1]), seconds=secs)
+        return out
+
+    def li(self, rays, keys, max_depth, seed, threads=8):
+        rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 8))
+        keys = np.ascontiguousarray(np.asarray(keys, np.uint32).reshape(-1, 2))
+        out = np.zeros((len(rays), 3), np.float32)
+        _pt_sigs().oracle_pt_li(self.h, rays.ctypes.data, keys.ctypes.data, len(rays), max_depth, seed, threads,
+                                out.ctypes.data)
+        return out
+
+
+def _bxdf(mat, mode, wo, wi=None, u=None):
+    rec = None if mat is None else material_record(mat)
+    wo = np.ascontiguousarray(np.asarray(wo, np.float32).reshape(-1, 3))
+    n = len(wo)
+    wi = None if wi is None else np.ascontiguousarray(np.asarray(wi, np.float32).reshape(-1, 3))
+    u = None if u is None else np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 7))
+    out = np.zeros((n, 7), np.float32)
+    _pt_sigs().oracle_bxdf(_ptr(rec), mode, n, _ptr(wo), _ptr(wi), _ptr(u), _ptr(out))
+    return out
+
+
+def bxdf_f(mat, wo, wi):
+    """f(wo, wi) * |cos wi| (SORT's convention) for arrays of local directions: (n, 3)."""
+    return _bxdf(mat, 0, wo, wi=wi)[:, 0:3]
+
+
+def bxdf_pdf(mat, wo, wi):
+    return _bxdf(mat, 1, wo, wi=wi)[:, 0]
+
+
+def bxdf_sample(mat, wo, u7):
+    """Returns f (n,3), wi (n,3), pdf (n,)."""
+    out = _bxdf(mat, 2, wo, u=u7)
+    return out[:, 0:3], out[:, 3:6], out[:, 6]

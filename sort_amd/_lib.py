@@ -37,6 +37,25 @@ class BuildOpts(ctypes.Structure):
                 ("max_leaf", ctypes.c_uint32), ("build_threads", ctypes.c_uint32)]
 
 
+class Material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("base", ctypes.c_float * 3), ("trans", ctypes.c_float * 3),
+                ("roughness", ctypes.c_float), ("ior", ctypes.c_float), ("eta", ctypes.c_float * 3),
+                ("k", ctypes.c_float * 3), ("metallic", ctypes.c_float), ("specular", ctypes.c_float),
+                ("specular_tint", ctypes.c_float), ("sheen", ctypes.c_float), ("sheen_tint", ctypes.c_float),
+                ("clearcoat", ctypes.c_float), ("clearcoat_gloss", ctypes.c_float)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("normals", ctypes.c_void_p), ("tangents", ctypes.c_void_p), ("tri_material", ctypes.c_void_p),
+                ("materials", ctypes.c_void_p), ("nmaterials", ctypes.c_uint32), ("quad_radiance", ctypes.c_void_p),
+                ("env_radiance", ctypes.c_float * 3)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("fov_y", ctypes.c_float)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -57,6 +76,11 @@ _SIGS = {
     "sortgpu_trace_closest": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_any": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_counters": (_I, [_P, _P, _U32, _I, _P, _P, _P, _P]),
+    "sortgpu_scene_create": (_I, [_P, ctypes.POINTER(SceneDesc), ctypes.POINTER(_P)]),
+    "sortgpu_scene_destroy": (None, [_P]),
+    "sortgpu_render": (_I, [_P, ctypes.POINTER(Camera), _U32, _U32, _U32, _U32, ctypes.c_uint64, _P, _P]),
+    "sortgpu_li": (_I, [_P, _P, _P, _U32, _U32, ctypes.c_uint64, _P, _P]),
+    "sortgpu_render_stats": (_I, [_P, _P]),
 }
 
 

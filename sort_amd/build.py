@@ -37,14 +37,15 @@ DEVICE = [
 ]
 
 HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp"]
-DEVICE_SOURCES = ["trace.hip"]
+# compiled as HIP: kernels, and host code that shares device record types with them
+DEVICE_SOURCES = ["trace.hip", "pt.hip", "pt_api.cpp"]
 
 
 def _compile(src: str) -> str:
     os.makedirs(OBJ, exist_ok=True)
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON
-    if src.endswith(".hip"):
+    if src in DEVICE_SOURCES:
         cmd += DEVICE + ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

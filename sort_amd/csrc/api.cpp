@@ -150,6 +150,15 @@ int sortgpu_accel_build(sortgpu_ctx* ctx, const float* pos, uint32_t nverts, con
         delete acc;
         return SORTGPU_ERR_ARG;
     }
+    try {
+        acc->pos.assign(pos, pos + 3 * (size_t)nverts);
+        acc->idx.assign(idx, idx + 3 * (size_t)ntris);
+        acc->quads.assign(quads, quads + nquads);
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "sortgpu_accel_build: host allocation failed");
+        delete acc;
+        return SORTGPU_ERR_OOM;
+    }
     TreeView& v = acc->view;
     std::memset(&v, 0, sizeof(v));
     v.root = acc->host.root_ref;
@@ -240,6 +249,7 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     a.tree = acc->view;
     a.rays = d_rays;
     a.n = n;
+    a.n_dev = nullptr;
     a.out = d_out;
     const uint32_t slot = acc->slot.fetch_add(1) % kCounterSlots;
     a.counter = acc->d_counters + slot * kCounterStride;

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/sortgpu.h"
 #include "bvh_build.h"
@@ -39,6 +40,10 @@ struct sortgpu_accel {
     unsigned long long* d_stats = nullptr;
     std::atomic<uint32_t> slot{0};
     size_t device_bytes = 0;
+    // host copies of the input geometry (shading and hit reconstruction need it by primitive id)
+    std::vector<float> pos;
+    std::vector<uint32_t> idx;
+    std::vector<sortgpu_quad> quads;
 };
 
 namespace sortgpu {

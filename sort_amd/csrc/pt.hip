@@ -1,0 +1,463 @@
+// Wavefront path tracer: SORT's PathTracing::Li bounce loop (integrator/pathtracing.cpp:44-258)
+// with EvaluateDirect's light-sampled + BSDF-sampled next-event estimation and power-2 MIS
+// (integrator/integratormethod.cpp:28-30,100-192), restructured as stages over fixed path slots:
+//
+//   raygen   camera rays (PerspectiveCamera::GenerateRay, camera/perspective.cpp:66-104) or caller rays
+//   extend   closest-hit traversal of the live queue (k_trace, device-side count)
+//   shade    emission at the first vertex, NEE (up to two shadow rays per vertex, written to
+//            per-path slots), BSDF continuation, Russian roulette, compaction of survivors by
+//            wave ballot + one atomic per wave
+//   shadow   any-hit traversal of the compacted shadow queue
+//   resolve  zero the contribution slot of every occluded shadow ray
+//   finalize fold the pending contributions, average each pixel's valid samples in sample order
+//            (image_evaluation.cpp:222-244: NaN/inf samples dropped from the mean)
+//
+// Path results do not depend on queue order: all randomness is keyed by (seed, pixel, sample,
+// bounce, call) and every accumulation into a path happens in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "pt_launch.h"
+#include "shading.h"
+#include "trace_launch.h"
+
+namespace sortgpu {
+
+constexpr int kPtBlock = 256;
+
+__device__ __forceinline__ float mis(float f, float g) { return (f * f) / (f * f + g * g); }
+
+// Distribution1D::SampleDiscrete (core/samplemethod.h:158-176).
+__device__ __forceinline__ int sample_discrete(const float* cdf, uint32_t count, float u, float* pdf) {
+    uint32_t lo = 0, hi = count + 1;  // lower_bound over cdf[0..count]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] < u) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t off = (u <= 0.0f) ? 0u : lo - 1u;
+    if (off == 0)
+        while (off < count && cdf[off + 1] == 0.0f) off++;
+    if (off == count) {
+        *pdf = 0.0f;
+        return 0;
+    }
+    *pdf = cdf[off + 1] - cdf[off];
+    return (int)off;
+}
+
+// Wave-aggregated append: every lane with `pred` gets a unique slot in [0, *counter).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void put_ray(float4* q, uint32_t i, V3 o, float tmin, V3 d, float tmax) {
+    q[2 * i] = make_float4(o.x, o.y, o.z, tmin);
+    q[2 * i + 1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+// ---- raygen ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kPtBlock) void k_raygen(const CamDev cam, uint32_t seed, uint32_t npix,
+                                                     uint32_t sample0, uint32_t npaths, PathState ps, Queues q) {
+    const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
+    if (p == 0) q.count[0] = npaths;
+    if (p >= npaths) return;
+    const uint32_t pix = p % npix;
+    const uint32_t s = sample0 + p / npix;
+    const uint32_t x = pix % cam.width, y = pix / cam.width;
+    const Rng4 j = rng4(seed, pix, s, kCameraBounce, 0);
+    // raster -> camera (inverse of Scale(W,H,1) Scale(.5,-.5,1) Translate(1,-1,0) Perspective(xs, ys))
+    const float rx = (float)x + j.a, ry = (float)y + j.b;
+    const float cx = 2.0f * rx / (float)cam.width - 1.0f;
+    const float cy = 1.0f - 2.0f * ry / (float)cam.height;
+    const V3 dc = normalize(v3(cx / cam.x_scale, cy / cam.y_scale, 1.0f));
+    const V3 d = cam.right * dc.x + cam.up * dc.y + cam.fwd * dc.z;
+    put_ray(q.ray[0], p, cam.eye, 0.0f, d, FLT_MAX);
+    q.pid[0][p] = p;
+    ps.T[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.c0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.c1[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.key[p] = pix;
+    ps.smp[p] = s;
+}
+
+// Caller-provided rays (the batched form of Integrator::Li): keys[2i] = pixel, keys[2i+1] = sample.
+__global__ __launch_bounds__(kPtBlock) void k_raygen_rays(const float4* rays, const uint32_t* keys, uint32_t n,
+                                                          PathState ps, Queues q) {
+    const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
+    if (p == 0) q.count[0] = n;
+    if (p >= n) return;
+    q.ray[0][2 * p] = rays[2 * p];
+    q.ray[0][2 * p + 1] = rays[2 * p + 1];
+    q.pid[0][p] = p;
+    ps.T[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.c0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.c1[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.key[p] = keys[2 * p];
+    ps.smp[p] = keys[2 * p + 1];
+}
+
+__global__ void k_zero_counts(uint32_t* count, int which) {
+    if (threadIdx.x == 0) {
+        count[which] = 0;
+        count[2] = 0;
+    }
+}
+
+__global__ void k_add_totals(const uint32_t* count, int ext_slot, unsigned long long* totals) {
+    if (threadIdx.x == 0) {
+        totals[0] += count[ext_slot];
+        totals[1] += count[2];
+    }
+}
+
+// Hit reconstruction at a triangle (setupIntersection, simd/simd_triangle.h:311-341): shading
+// normal and tangent interpolated with w = 1 - u - v.
+struct Surf {
+    V3 p, n, t;
+    uint32_t mat;
+    int32_t light;  // light index when the hit primitive is an area light
+};
+
+__device__ __forceinline__ V3 load3(const float* a, uint32_t i) { return v3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+
+__device__ __forceinline__ Surf surface(const SceneDev& sc, V3 o, V3 d, float t, uint32_t prim, float u, float v) {
+    Surf s;
+    s.p = o + t * d;
+    s.light = -1;
+    if (prim < sc.ntris) {
+        const uint32_t i0 = sc.idx[3 * prim], i1 = sc.idx[3 * prim + 1], i2 = sc.idx[3 * prim + 2];
+        const float w = 1.0f - u - v;
+        if (sc.nrm) {
+            s.n = normalize(w * load3(sc.nrm, i0) + u * load3(sc.nrm, i1) + v * load3(sc.nrm, i2));
+        } else {
+            const V3 p0 = load3(sc.pos, i0), p1 = load3(sc.pos, i1), p2 = load3(sc.pos, i2);
+            s.n = normalize(cross(p1 - p0, p2 - p0));
+        }
+        if (sc.tan) {
+            s.t = normalize(w * load3(sc.tan, i0) + u * load3(sc.tan, i1) + v * load3(sc.tan, i2));
+        } else {
+            s.t = normalize(load3(sc.pos, i1) - load3(sc.pos, i0));
+        }
+        s.mat = sc.tri_mat ? sc.tri_mat[prim] : 0u;
+    } else {
+        const uint32_t qi = prim - sc.ntris;
+        const int32_t li = sc.quad_light[qi];
+        const LightRec& l = sc.lights[li];
+        s.n = quad_normal(l);                      // TransformNormal(DIR_UP)
+        s.t = v3(l.mat[2], l.mat[6], l.mat[10]);  // TransformVector((0,0,1))
+        s.mat = sc.nmats;                          // default material: white Lambert
+        s.light = li;
+    }
+    return s;
+}
+
+// Shading frame (ScatteringEvent ctor, scatteringevent.cpp:44-49).
+struct Frame {
+    V3 t, n, b;
+};
+__device__ __forceinline__ Frame make_frame(V3 n_in, V3 tangent) {
+    Frame f;
+    f.n = normalize(n_in);
+    f.b = normalize(cross(f.n, tangent));
+    f.t = normalize(cross(f.b, f.n));
+    return f;
+}
+__device__ __forceinline__ V3 to_local(const Frame& f, V3 v) { return v3(dot(v, f.t), dot(v, f.n), dot(v, f.b)); }
+__device__ __forceinline__ V3 to_world(const Frame& f, V3 v) {
+    return v3(v.x * f.t.x + v.y * f.n.x + v.z * f.b.x, v.x * f.t.y + v.y * f.n.y + v.z * f.b.y,
+              v.x * f.t.z + v.y * f.n.z + v.z * f.b.z);
+}
+
+// ---- shade -------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__ scenes, uint32_t seed, uint32_t bounce,
+                                                    uint32_t max_depth, int cur, PathState ps, Queues q) {
+    const SceneDev& sc = scenes[0];
+    const uint32_t n = q.count[cur];
+    const int nxt = cur ^ 1;
+    for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false, sh0 = false, sh1 = false;
+        V3 ro = v3(0, 0, 0), rd = v3(0, 0, 0), so = v3(0, 0, 0), s0d = v3(0, 0, 0), s1d = v3(0, 0, 0);
+        float s0max = 0.f, s1max = 0.f;
+        uint32_t p = 0;
+        if (i < n) {
+            p = q.pid[cur][i];
+            const float4 h = q.hit[i];
+            const float4 r0 = q.ray[cur][2 * i], r1 = q.ray[cur][2 * i + 1];
+            const V3 o = v3(r0.x, r0.y, r0.z), d = v3(r1.x, r1.y, r1.z);
+            float4 Lp = ps.L[p];
+            V3 L = v3(Lp.x, Lp.y, Lp.z);
+            {   // fold the previous vertex's resolved NEE contributions
+                const float4 a = ps.c0[p], b = ps.c1[p];
+                L = L + v3(a.x, a.y, a.z) + v3(b.x, b.y, b.z);
+            }
+            float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
+            const float4 Tp = ps.T[p];
+            V3 T = v3(Tp.x, Tp.y, Tp.z);
+            const uint32_t prim = __float_as_uint(h.y);
+            if (prim == 0xFFFFFFFFu) {
+                if (bounce == 0 && sc.env_light >= 0) L = L + sc.env_L;  // Scene::Le on a camera miss
+            } else {
+                const uint32_t key = ps.key[p], smp = ps.smp[p];
+                const Surf s = surface(sc, o, d, h.x, prim, h.z, h.w);
+                if (bounce == 0 && s.light >= 0) {  // SurfaceInteraction::Le -> AreaLight::Le
+                    const LightRec& l = sc.lights[s.light];
+                    const float c = saturate(dot(-d, s.n));
+                    if (c != 0.0f) L = L + l.L;
+                }
+                const MatRec m = sc.mats[s.mat];
+                const Frame fr = make_frame(s.n, s.t);
+                const V3 wo_w = -d;
+                const V3 wo = to_local(fr, wo_w);
+                const Rng4 ra = rng4(seed, key, smp, bounce, 0);
+                const Rng4 rb = rng4(seed, key, smp, bounce, 1);
+                const Rng4 rc = rng4(seed, key, smp, bounce, 2);
+                const Rng4 rf_nee = rng4(seed, key, smp, bounce, 3);
+                const Rng4 rf_cont = rng4(seed, key, smp, bounce, 4);
+                // ---- next-event estimation (EvaluateDirect) ----
+                float pick_pdf = 0.0f;
+                const int li = sc.nlights ? sample_discrete(sc.light_cdf, sc.nlights, ra.a, &pick_pdf) : 0;
+                if (pick_pdf > 0.0f) {
+                    const LightRec& l = sc.lights[li];
+                    V3 wi, Li = v3(0, 0, 0);
+                    float light_pdf = 0.0f, smax = 0.0f;
+                    if (l.kind == 0) {  // AreaLight::sample_l -> Quad::Sample_l
+                        const float uu = 2.0f * ra.b - 1.0f, vv = 2.0f * ra.c - 1.0f;
+                        const V3 lp = xform_point3(l.mat, v3(l.half_x * uu, 0.0f, l.half_y * vv));
+                        const V3 nl = xform_vec3(l.mat, v3(0.0f, 1.0f, 0.0f));
+                        const V3 delta = lp - s.p;
+                        wi = normalize(delta);
+                        const float dd = dot(-wi, nl);
+                        light_pdf = dd <= 0.0f ? 0.0f : dot(delta, delta) / (l.area * dd);
+                        const float len = length(delta);
+                        smax = len - 0.01f;
+                        Li = l.L;
+                    } else {  // AmbientLight / SkyLight::sample_l
+                        wi = uniform_sample_sphere(ra.b, ra.c);
+                        light_pdf = kInvFourPi;
+                        smax = FLT_MAX;
+                        Li = l.L;
+                    }
+                    if (light_pdf > 0.0f && !is_black(Li)) {
+                        const V3 wil = to_local(fr, wi);
+                        const V3 f = mat_f(m, wo, wil);
+                        if (!is_black(f)) {
+                            const float bp = mat_pdf(m, wo, wil);
+                            const float w = mis(light_pdf, bp);
+                            const V3 c = T * (Li * f * w / light_pdf) / pick_pdf;
+                            c0 = make_float4(c.x, c.y, c.z, 0.f);
+                            sh0 = true;
+                            so = s.p;
+                            s0d = wi;
+                            s0max = smax;
+                        }
+                    }
+                    // BSDF-sampled half with a separate BsdfSample (integratormethod.cpp:150-189)
+                    BsdfSampleOut bs = mat_sample(m, wo, rb.b, rb.c, rf_nee);
+                    if (bs.pdf == 0.0f) bs.f = v3(0, 0, 0);
+                    if (!is_black(bs.f) && bs.pdf != 0.0f) {
+                        const V3 wi2 = to_world(fr, bs.wi);
+                        float lp2 = 0.0f, tl = -1.0f;
+                        V3 Li2 = v3(0, 0, 0);
+                        if (l.kind == 0) {  // Shape::Pdf + AreaLight::Le(ray)
+                            tl = quad_hit(l, s.p, wi2);
+                            if (tl >= 0.0f) {
+                                const V3 nl = quad_normal(l);
+                                const V3 hp = s.p + tl * wi2;
+                                const V3 delta = s.p - hp;
+                                const float dn = saturate(dot(normalize(delta), nl));
+                                lp2 = dn <= 0.0f ? 0.0f : dot(delta, delta) / (l.area * dn);
+                                if (saturate(dot(-wi2, nl)) != 0.0f) Li2 = l.L;
+                            }
+                        } else {
+                            lp2 = kInvFourPi;
+                            tl = FLT_MAX;
+                            Li2 = l.L;
+                        }
+                        if (lp2 > 0.0f && tl >= 0.0f && !is_black(Li2)) {
+                            const float w2 = mis(bs.pdf, lp2);
+                            const V3 c = T * (Li2 * bs.f * w2 / bs.pdf) / pick_pdf;
+                            c1 = make_float4(c.x, c.y, c.z, 0.f);
+                            sh1 = true;
+                            s1d = wi2;
+                            s1max = tl - 0.001f;
+                            so = s.p;
+                        }
+                    }
+                }
+                // ---- continuation (pathtracing.cpp:179-249) ----
+                BsdfSampleOut cs = mat_sample(m, wo, rc.b, rc.c, rf_cont);
+                if (cs.pdf == 0.0f) cs.f = v3(0, 0, 0);
+                if (!is_black(cs.f) && cs.pdf != 0.0f) {
+                    T = T * (cs.f / cs.pdf);
+                    if (luminance(T) != 0.0f) {
+                        cont = true;
+                        if (bounce > 3 && maxc(T) < 0.1f) {
+                            const float q_rr = fmaxf(0.05f, 1.0f - maxc(T));
+                            if (rc.d < q_rr) cont = false;
+                            else T = T / (1.0f - q_rr);
+                        }
+                        if (bounce + 1 >= max_depth) cont = false;
+                        ro = s.p;
+                        rd = to_world(fr, cs.wi);
+                    }
+                }
+            }
+            ps.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+            ps.T[p] = make_float4(T.x, T.y, T.z, 0.f);
+            ps.c0[p] = c0;
+            ps.c1[p] = c1;
+        }
+        // compaction: survivors and shadow rays append with one atomic per wave
+        const uint32_t ci = wave_append(cont, &q.count[nxt]);
+        if (cont) {
+            put_ray(q.ray[nxt], ci, ro, 0.0001f, rd, FLT_MAX);
+            q.pid[nxt][ci] = p;
+        }
+        const uint32_t si0 = wave_append(sh0, &q.count[2]);
+        if (sh0) {
+            put_ray(q.sray, si0, so, 0.01f, s0d, s0max);
+            q.ssid[si0] = 2 * p;
+        }
+        const uint32_t si1 = wave_append(sh1, &q.count[2]);
+        if (sh1) {
+            put_ray(q.sray, si1, so, 0.001f, s1d, s1max);
+            q.ssid[si1] = 2 * p + 1;
+        }
+    }
+}
+
+// ---- resolve -----------------------------------------------------------------------------------
+__global__ __launch_bounds__(kPtBlock) void k_resolve(PathState ps, Queues q) {
+    const uint32_t n = q.count[2];
+    for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
+        if (q.socc[i]) {
+            const uint32_t slot = q.ssid[i];
+            float4* c = (slot & 1u) ? ps.c1 : ps.c0;
+            c[slot >> 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+}
+
+// ---- finalize ----------------------------------------------------------------------------------
+// Pixel i owns paths i, i + npix, ... (one per sample of the batch, in sample order).
+__global__ __launch_bounds__(kPtBlock) void k_finalize(PathState ps, uint32_t npix, uint32_t nsamp, float4* film) {
+    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+    if (i >= npix) return;
+    float4 acc = film[i];
+    for (uint32_t s = 0; s < nsamp; s++) {
+        const uint32_t p = s * npix + i;
+        const float4 l = ps.L[p], a = ps.c0[p], b = ps.c1[p];
+        const float r = l.x + a.x + b.x, g = l.y + a.y + b.y, bb = l.z + a.z + b.z;
+        if (isfinite(r) && isfinite(g) && isfinite(bb)) {
+            acc.x += r;
+            acc.y += g;
+            acc.z += bb;
+            acc.w += 1.0f;
+        }
+    }
+    film[i] = acc;
+}
+
+// Per-ray radiance for the caller-ray form.
+__global__ __launch_bounds__(kPtBlock) void k_finalize_rays(PathState ps, uint32_t n, float* out) {
+    const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
+    if (p >= n) return;
+    const float4 l = ps.L[p], a = ps.c0[p], b = ps.c1[p];
+    out[3 * p] = l.x + a.x + b.x;
+    out[3 * p + 1] = l.y + a.y + b.y;
+    out[3 * p + 2] = l.z + a.z + b.z;
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_resolve_film(const float4* film, uint32_t npix, float* rgb) {
+    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+    if (i >= npix) return;
+    const float4 a = film[i];
+    const float inv = a.w > 0.0f ? 1.0f / a.w : 0.0f;
+    rgb[3 * i] = a.x * inv;
+    rgb[3 * i + 1] = a.y * inv;
+    rgb[3 * i + 2] = a.z * inv;
+}
+
+__global__ void k_zero_f4(float4* a, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---- host driver -------------------------------------------------------------------------------
+static inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + kPtBlock - 1) / kPtBlock); }
+
+hipError_t pt_zero_f4(float4* a, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    unsigned b = blocks_for(n);
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_zero_f4, dim3(b), dim3(kPtBlock), 0, s, a, n);
+    return hipGetLastError();
+}
+
+// Runs the bounce loop for the paths already in queue 0 (count q.count[0] = npaths).
+hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
+    const unsigned shade_blocks = std::min<unsigned>(blocks_for(npaths), (unsigned)L.num_cus * 8u);
+    hipError_t e;
+    for (uint32_t b = 0; b < L.max_depth; b++) {
+        const int cur = (int)(b & 1u), nxt = cur ^ 1;
+        TraceArgs ta = L.trace;
+        ta.rays = L.q.ray[cur];
+        ta.n = npaths;
+        ta.n_dev = &L.q.count[cur];
+        ta.out = L.q.hit;
+        if ((e = launch_trace(ta, false, false, s, L.num_cus)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, L.q.count, nxt);
+        hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.seed, b, L.max_depth, cur,
+                           L.ps, L.q);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        TraceArgs sa = L.trace;
+        sa.rays = L.q.sray;
+        sa.n = 2 * npaths;
+        sa.n_dev = &L.q.count[2];
+        sa.out = L.q.socc;
+        if ((e = launch_trace(sa, true, false, s, L.num_cus)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_resolve, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.ps, L.q);
+        hipLaunchKernelGGL(k_add_totals, dim3(1), dim3(64), 0, s, L.q.count, cur, L.q.totals);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t pt_render_batch(const PtLaunch& L, const CamDev& cam, uint32_t sample0, uint32_t nsamp, float4* film,
+                           hipStream_t s) {
+    const uint32_t npix = cam.width * cam.height;
+    const uint32_t npaths = npix * nsamp;
+    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(npaths)), dim3(kPtBlock), 0, s, cam, L.seed, npix, sample0, npaths,
+                       L.ps, L.q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = pt_bounces(L, npaths, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(npix)), dim3(kPtBlock), 0, s, L.ps, npix, nsamp, film);
+    return hipGetLastError();
+}
+
+hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStream_t s) {
+    hipLaunchKernelGGL(k_resolve_film, dim3(blocks_for(npix)), dim3(kPtBlock), 0, s, film, npix, rgb);
+    return hipGetLastError();
+}
+
+hipError_t pt_li_rays(const PtLaunch& L, const float4* rays, const uint32_t* keys, uint32_t n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_raygen_rays, dim3(blocks_for(n)), dim3(kPtBlock), 0, s, rays, keys, n, L.ps, L.q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = pt_bounces(L, n, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize_rays, dim3(blocks_for(n)), dim3(kPtBlock), 0, s, L.ps, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace sortgpu

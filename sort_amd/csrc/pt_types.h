@@ -1,0 +1,56 @@
+// Path-tracer data shared by the host driver and the kernels.
+#pragma once
+#include <stdint.h>
+
+#include "layout.h"
+#include "shading.h"
+
+namespace sortgpu {
+
+// One renderable scene resident in HBM: the tree plus everything shading reads.
+struct SceneDev {
+    TreeView tree;
+    const uint32_t* idx;       // ntris x 3
+    const float* pos;          // nverts x 3
+    const float* nrm;          // nverts x 3 shading normals, or null
+    const float* tan;          // nverts x 3 tangents, or null
+    const uint32_t* tri_mat;   // ntris, or null (material 0)
+    const MatRec* mats;        // nmats + 1 (the last is the default white Lambert)
+    const LightRec* lights;    // nlights
+    const float* light_cdf;    // nlights + 1
+    const int32_t* quad_light; // nquads: light index of each quad
+    uint32_t ntris, nquads, nlights, nmats;
+    int32_t env_light;         // index of the environment light, or -1
+    V3 env_L;
+};
+
+struct CamDev {
+    V3 eye, right, up, fwd;
+    float x_scale, y_scale;  // 1/tan(fov/2) / aspect, 1/tan(fov/2)
+    uint32_t width, height;
+};
+
+// Path slots of one batch (fixed for the batch; index = path id).
+struct PathState {
+    float4* T;        // throughput
+    float4* L;        // radiance so far
+    float4* c0;       // pending light-sampled NEE contribution of the last vertex
+    float4* c1;       // pending BSDF-sampled NEE contribution of the last vertex
+    uint32_t* key;    // per path: scene-local pixel index (RNG key)
+    uint32_t* smp;    // per path: sample index (RNG counter)
+    uint32_t* req;    // per path: request id (served form), 0 otherwise
+};
+
+struct Queues {
+    float4* ray[2];     // 2 x float4 per entry
+    uint32_t* pid[2];
+    float4* hit;        // per entry of the current queue
+    float4* sray;       // shadow rays
+    uint32_t* ssid;     // shadow slot: 2 * path + k
+    uint8_t* socc;      // shadow occlusion
+    uint32_t* count;    // [0],[1] extension queue counts, [2] shadow count, [3] spare,
+                        // [4] total extension rays, [5] total shadow rays (64-bit pairs below)
+    unsigned long long* totals;  // [0] extension rays, [1] shadow rays
+};
+
+}  // namespace sortgpu

@@ -37,6 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
+    const uint32_t n_rays = a.n_dev ? *a.n_dev : a.n;
     const NodeRec<W>* __restrict__ nodes = static_cast<const NodeRec<W>*>(a.tree.nodes);
     const PrimRec* __restrict__ prims = a.tree.prims;
 
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
             base = __shfl(base, leader);
             if (idle && more) {
                 const uint32_t my = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-                if (my < a.n) {
+                if (my < n_rays) {
                     ray_id = my;
                     const float4* rp = reinterpret_cast<const float4*>(a.rays) + 2 * (size_t)my;
                     const float4 q0 = rp[0], q1 = rp[1];

@@ -10,7 +10,8 @@ namespace sortgpu {
 struct TraceArgs {
     TreeView tree;
     const void* rays;              // n x 32 B ray records (device)
-    uint32_t n;
+    uint32_t n;                    // ray count, or the capacity when n_dev is set
+    const uint32_t* n_dev;         // optional device-side ray count (read at kernel start)
     void* out;                     // closest: n x 16 B hits; any: n x u8 (device)
     uint32_t* counter;             // work counter (device), reset by the launch
     uint32_t* error;               // error flags (device): bit0 = stack overflow

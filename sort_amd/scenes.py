@@ -100,25 +100,128 @@ def tris_to_mesh(tris):
     return np.ascontiguousarray(pos), np.ascontiguousarray(idx)
 
 
-def cornell_sphere():
-    """Returns dict(pos, idx, quads, mat_id (per triangle), groups) for the Cornell + sphere
-    scene. Triangle order: room walls (12), pedestal (12), sphere (4,032)."""
+def face_normals_per_vertex(pos, idx):
+    """Per-vertex normals equal to each (unshared-vertex) triangle's CCW face normal."""
+    p0, p1, p2 = pos[idx[:, 0]], pos[idx[:, 1]], pos[idx[:, 2]]
+    n = np.cross(p1 - p0, p2 - p0).astype(np.float32)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    ln[ln == 0] = 1.0
+    n = (n / ln).astype(np.float32)
+    out = np.zeros_like(pos)
+    out[idx[:, 0]] = n
+    out[idx[:, 1]] = n
+    out[idx[:, 2]] = n
+    return out
+
+
+def cornell_sphere(light_radiance=(17.0, 17.0, 17.0), sphere_material=None):
+    """Closed room [0,1]^3 (6 walls, inward normals) + a 6-face pedestal = 12 axis-aligned quads
+    (24 triangles), a 64x32 UV sphere (4,032 triangles, radius 0.25 at the centre, smooth
+    normals) and one 0.25 x 0.25 area light at y = 0.999 facing -y.
+
+    Triangle order: room walls (12), pedestal (12), sphere (4,032). Materials: 0 white Lambert
+    0.7, 1 red (0.7, 0.2, 0.2) on the -x wall, 2 green (0.2, 0.7, 0.2) on the +x wall, 3 the
+    pedestal (white 0.7), 4 the sphere (white Lambert 0.7 unless `sphere_material` is given).
+    The camera sits inside the room by the z = 0 wall looking at +z."""
+    from .integrator import lambert
     room = axis_box_faces((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
-    # room faces must face inward: flip the winding of every room triangle
-    room = [t[[0, 2, 1]] for t in room]
+    room = [t[[0, 2, 1]] for t in room]  # inward-facing walls
     pedestal = axis_box_faces((0.35, 0.0, 0.35), (0.65, 0.2, 0.65))
     sphere = uv_sphere((0.5, 0.5, 0.5), 0.25)
     tris = room + pedestal + sphere
     pos, idx = tris_to_mesh(tris)
-    # materials: 0 white wall, 1 red (-x wall), 2 green (+x wall), 3 pedestal, 4 sphere
+    nrm = face_normals_per_vertex(pos, idx)
+    sv = idx[24:].reshape(-1)
+    d = pos[sv] - np.array([0.5, 0.5, 0.5], np.float32)
+    nrm[sv] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
     mat = np.zeros(len(tris), np.uint32)
     mat[4:6] = 1   # -x wall
     mat[6:8] = 2   # +x wall
     mat[12:24] = 3
     mat[24:] = 4
+    materials = [lambert((0.7, 0.7, 0.7)), lambert((0.7, 0.2, 0.2)), lambert((0.2, 0.7, 0.2)),
+                 lambert((0.7, 0.7, 0.7)), sphere_material or lambert((0.7, 0.7, 0.7))]
     light = translate_quad((0.5, 0.999, 0.5), 0.25, facing_down=True)
-    return {"pos": pos, "idx": idx, "quads": [light], "mat": mat,
-            "ntris": len(tris), "counts": {"room": 12, "pedestal": 12, "sphere": len(sphere)}}
+    return {"pos": pos, "idx": idx, "quads": [light], "normals": nrm, "tri_material": mat,
+            "materials": materials, "quad_radiance": np.array([light_radiance], np.float32),
+            "env": (0.0, 0.0, 0.0), "ntris": len(tris),
+            "camera": dict(eye=(0.5, 0.5, 0.02), target=(0.5, 0.5, 1.0), up=(0.0, 1.0, 0.0), fov_y_deg=70.0),
+            "counts": {"room": 12, "pedestal": 12, "sphere": len(sphere)}}
+
+
+def icosphere(level=5):
+    """Unit icosphere: the icosahedron (level 1) subdivided level-1 times: 20 * 4^(level-1)
+    triangles (5,120 at level 5), shared vertices, outward CCW winding."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(x, np.float64) / np.linalg.norm(x) for x in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level - 1):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[k] = len(verts) - 1
+            return cache[k]
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts, np.float32), np.array(faces, np.uint32)
+
+
+def spheres_scene(n_spheres=400, level=5, seed=SEED_PT):
+    """Ground quad + n seeded icospheres (radii U[0.05, 0.4], resting on the ground) baked into
+    one mesh; materials drawn 50% Lambert / 30% GGX (roughness U[0.05, 0.8]) / 20% dielectric
+    (IOR 1.5); a constant environment and 4 quad lights facing down."""
+    from .integrator import dielectric, ggx, lambert
+    g = rng(seed)
+    uv, uf = icosphere(level)
+    nv, nf = len(uv), len(uf)
+    G = 6.0
+    ground_p = np.array([[-G, 0, -G], [G, 0, -G], [G, 0, G], [-G, 0, G]], np.float32)
+    ground_i = np.array([[0, 2, 1], [0, 3, 2]], np.uint32)  # CCW seen from +y
+    radii = g.uniform(0.05, 0.4, n_spheres).astype(np.float32)
+    cx = g.uniform(-4.5, 4.5, n_spheres).astype(np.float32)
+    cz = g.uniform(-4.5, 4.5, n_spheres).astype(np.float32)
+    pos = np.empty((4 + n_spheres * nv, 3), np.float32)
+    nrm = np.empty_like(pos)
+    idx = np.empty((2 + n_spheres * nf, 3), np.uint32)
+    pos[:4] = ground_p
+    nrm[:4] = (0, 1, 0)
+    idx[:2] = ground_i
+    for i in range(n_spheres):
+        c = np.array([cx[i], radii[i], cz[i]], np.float32)
+        pos[4 + i * nv: 4 + (i + 1) * nv] = c + radii[i] * uv
+        nrm[4 + i * nv: 4 + (i + 1) * nv] = uv
+        idx[2 + i * nf: 2 + (i + 1) * nf] = uf + np.uint32(4 + i * nv)
+    kind = g.random(n_spheres)
+    rough = g.uniform(0.05, 0.8, n_spheres).astype(np.float32)
+    albedo = g.uniform(0.2, 0.9, (n_spheres, 3)).astype(np.float32)
+    materials = [lambert((0.6, 0.6, 0.6))]
+    sphere_mat = np.empty(n_spheres, np.uint32)
+    for i in range(n_spheres):
+        sphere_mat[i] = len(materials)
+        if kind[i] < 0.5:
+            materials.append(lambert(tuple(albedo[i])))
+        elif kind[i] < 0.8:
+            materials.append(ggx(tuple(albedo[i]), float(rough[i])))
+        else:
+            materials.append(dielectric((1, 1, 1), (1, 1, 1), roughness=float(rough[i]) * 0.25, ior=1.5))
+    tri_mat = np.zeros(len(idx), np.uint32)
+    tri_mat[2:] = np.repeat(sphere_mat, nf)
+    lights = [translate_quad((x, 3.5, z), 1.0, facing_down=True) for x, z in [(-2.5, -2.5), (2.5, -2.5), (-2.5, 2.5), (2.5, 2.5)]]
+    return {"pos": pos, "idx": idx, "quads": lights, "normals": nrm, "tri_material": tri_mat,
+            "materials": materials, "quad_radiance": np.full((4, 3), 6.0, np.float32), "env": (0.25, 0.3, 0.35),
+            "ntris": len(idx),
+            "camera": dict(eye=(0.0, 3.0, 9.0), target=(0.0, 0.5, 0.0), up=(0.0, 1.0, 0.0), fov_y_deg=45.0)}
 
 
 def soup(n_tris: int = 1 << 20, seed: int = SEED_SOUP):
