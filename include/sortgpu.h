@@ -123,6 +123,38 @@ int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint
                            sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4],
                            void* stream);
 
+/* ---- tuning (no counterpart in SORT: the knobs of the persistent traversal kernels) ---- */
+
+/* 0 in any field keeps the default. refill = idle lanes of a wave that trigger taking new rays
+ * (default 16, 1..64); lds_entries = traversal-stack entries per lane kept in LDS (8 or 16,
+ * default 16); blocks_per_cu = cap on resident workgroups per CU (default: occupancy limit). */
+typedef struct sortgpu_tuning {
+    uint32_t refill;
+    uint32_t lds_entries;
+    uint32_t blocks_per_cu;
+    uint32_t reserved;
+} sortgpu_tuning;
+
+/* Applies to every later query, render and pool trace of the context. */
+int sortgpu_ctx_set_tuning(sortgpu_ctx* ctx, const sortgpu_tuning* tuning);
+
+/* ---- batched form: many scenes' rays in one launch over a concatenated tree pool ----
+ * (no single counterpart in SORT, which traces one ray against one scene at a time through
+ * Accelerator::GetIntersect; each pooled tree is exactly the accelerator it was built as) */
+
+typedef struct sortgpu_pool sortgpu_pool;
+
+/* Copies the records of n accelerators (one width, built on any context, host-only included)
+ * into one device allocation, with a device table of their tree views. */
+int sortgpu_pool_create(sortgpu_ctx* ctx, sortgpu_accel* const* accels, uint32_t n, sortgpu_pool** out);
+void sortgpu_pool_destroy(sortgpu_pool* pool);
+/* out[0..3] = scenes, width, device bytes, deepest stack. */
+int sortgpu_pool_info(const sortgpu_pool* pool, uint64_t out[4]);
+/* One persistent launch over rays of many scenes: d_ray_scene[i] = index of ray i's scene in
+ * the pool. any = 0: d_out is n sortgpu_hit; any = 1: n uint8 occlusion flags. */
+int sortgpu_pool_trace(sortgpu_pool* pool, const sortgpu_ray* d_rays, const uint32_t* d_ray_scene, uint32_t n,
+                       int any, void* d_out, void* stream);
+
 /* ---- path tracing (PathTracing::Li, integrator/pathtracing.cpp:37-258) ---- */
 
 #define SORTGPU_MAT_LAMBERT 0    /* Lambert (scatteringevent/bsdf/lambert.cpp)                    */

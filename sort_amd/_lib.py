@@ -37,6 +37,11 @@ class BuildOpts(ctypes.Structure):
                 ("max_leaf", ctypes.c_uint32), ("build_threads", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("refill", ctypes.c_uint32), ("lds_entries", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
 class Material(ctypes.Structure):
     _fields_ = [("type", ctypes.c_uint32), ("base", ctypes.c_float * 3), ("trans", ctypes.c_float * 3),
                 ("roughness", ctypes.c_float), ("ior", ctypes.c_float), ("eta", ctypes.c_float * 3),
@@ -76,6 +81,11 @@ _SIGS = {
     "sortgpu_trace_closest": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_any": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_counters": (_I, [_P, _P, _U32, _I, _P, _P, _P, _P]),
+    "sortgpu_ctx_set_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
+    "sortgpu_pool_create": (_I, [_P, _P, _U32, ctypes.POINTER(_P)]),
+    "sortgpu_pool_destroy": (None, [_P]),
+    "sortgpu_pool_info": (_I, [_P, _P]),
+    "sortgpu_pool_trace": (_I, [_P, _P, _P, _U32, _I, _P, _P]),
     "sortgpu_scene_create": (_I, [_P, ctypes.POINTER(SceneDesc), ctypes.POINTER(_P)]),
     "sortgpu_scene_destroy": (None, [_P]),
     "sortgpu_render": (_I, [_P, ctypes.POINTER(Camera), _U32, _U32, _U32, _U32, ctypes.c_uint64, _P, _P]),

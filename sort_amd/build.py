@@ -36,7 +36,7 @@ DEVICE = [
     "-munsafe-fp-atomics",
 ]
 
-HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp"]
+HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp", "pool.cpp"]
 # compiled as HIP: kernels, and host code that shares device record types with them
 DEVICE_SOURCES = ["trace.hip", "pt.hip", "pt_api.cpp"]
 

@@ -41,6 +41,12 @@ int hip_fail(sortgpu_ctx* ctx, hipError_t e, const char* what) {
     return e == hipErrorOutOfMemory ? SORTGPU_ERR_OOM : SORTGPU_ERR_HIP;
 }
 
+void apply_tuning(const sortgpu_ctx* ctx, TraceArgs& a) {
+    a.refill = ctx->tuning.refill;
+    a.lds_entries = ctx->tuning.lds_entries;
+    a.blocks_per_cu = ctx->tuning.blocks_per_cu;
+}
+
 int ctx_bind(sortgpu_ctx* ctx) {
     int cur = -1;
     hipError_t e = hipGetDevice(&cur);
@@ -97,6 +103,19 @@ int sortgpu_ctx_create(int device, sortgpu_ctx** out) {
 }
 
 void sortgpu_ctx_destroy(sortgpu_ctx* ctx) { delete ctx; }
+
+int sortgpu_ctx_set_tuning(sortgpu_ctx* ctx, const sortgpu_tuning* t) {
+    if (!ctx || !t) {
+        set_error(ctx, "sortgpu_ctx_set_tuning: null argument");
+        return SORTGPU_ERR_ARG;
+    }
+    if (t->refill > 64 || (t->lds_entries != 0 && t->lds_entries != 8 && t->lds_entries != 16)) {
+        set_error(ctx, "sortgpu_ctx_set_tuning: refill must be <= 64 and lds_entries 0, 8 or 16");
+        return SORTGPU_ERR_ARG;
+    }
+    ctx->tuning = *t;
+    return SORTGPU_OK;
+}
 
 const char* sortgpu_last_error(const sortgpu_ctx* ctx) {
     if (ctx) return ctx->err.c_str();
@@ -245,7 +264,7 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     int rc = ctx_bind(ctx);
     if (rc != SORTGPU_OK) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    TraceArgs a;
+    TraceArgs a{};
     a.tree = acc->view;
     a.rays = d_rays;
     a.n = n;
@@ -255,6 +274,7 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     a.counter = acc->d_counters + slot * kCounterStride;
     a.error = acc->d_error;
     a.stats = nullptr;
+    apply_tuning(ctx, a);
     if (stats) {
         hipError_t e = hipMemsetAsync(acc->d_stats, 0, 4 * sizeof(unsigned long long), s);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(stats)");

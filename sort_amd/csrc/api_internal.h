@@ -17,6 +17,7 @@ struct sortgpu_ctx {
     int num_cus = 0;
     size_t hbm_bytes = 0;
     char arch[64] = {0};
+    sortgpu_tuning tuning{};
     std::mutex mu;
     std::string err;
 };
@@ -51,5 +52,7 @@ void set_error(sortgpu_ctx* ctx, const char* fmt, ...);
 int hip_fail(sortgpu_ctx* ctx, hipError_t e, const char* what);
 int ctx_bind(sortgpu_ctx* ctx);
 int upload_accel(sortgpu_accel* acc);
+// Copies the context's tuning knobs into a launch.
+void apply_tuning(const sortgpu_ctx* ctx, struct TraceArgs& a);
 void free_accel_device(sortgpu_accel* acc);
 }  // namespace sortgpu

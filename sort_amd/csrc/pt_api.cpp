@@ -97,6 +97,7 @@ PtLaunch make_launch(sortgpu_scene* sc, uint32_t max_depth, uint64_t seed) {
     L.trace.counter = acc->d_counters + slot * kCounterStride;
     L.trace.error = acc->d_error;
     L.trace.stats = nullptr;
+    apply_tuning(acc->ctx, L.trace);
     L.scenes = sc->d_scene;
     L.ps = sc->ps;
     L.q = sc->q;

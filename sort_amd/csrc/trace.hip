@@ -1,12 +1,16 @@
-// Closest-hit and any-hit traversal kernels over a flattened Fast-BVH (8- or 4-wide).
+// Closest-hit and any-hit traversal kernels over flattened Fast-BVHs (8- or 4-wide).
 //
 // Execution model: persistent waves, one ray per lane. A lane whose stack empties writes its
 // result and, together with the other idle lanes of its wave, takes the next rays from a
-// device work counter (one atomic per refill, taken when at least kRefill lanes are idle or the
-// whole wave is), so long rays do not hold a wave's other 63 lanes idle.
+// device work counter (one atomic per refill, taken when at least `refill` lanes are idle or
+// the whole wave is), so long rays do not hold a wave's other 63 lanes idle.
 //
-// Traversal stack: entries (child reference, entry distance). The top kStackLds entries of
-// every lane live in LDS ([entry][thread] so a wave's accesses to one entry hit 64 consecutive
+// Multi-scene form (MULTI): every ray names its scene (ray_scene[i]); a lane loads that
+// scene's tree view (root, box, record pointers into the concatenated pool) when it takes the
+// ray, so rays of any number of scenes share one launch and one work queue.
+//
+// Traversal stack: entries (child reference, entry distance). The top LDS entries of every
+// lane live in LDS ([entry][thread] so a wave's accesses to one entry hit 64 consecutive
 // dwords); deeper entries spill to a private per-lane array in scratch. The host picks the
 // spill depth from the tree's exact worst case (W-1)*(depth-1)+1, so no push can go past it;
 // the kernel still bounds-checks every push and raises an error flag instead of writing.
@@ -24,22 +28,23 @@
 namespace sortgpu {
 
 constexpr int kBlock = 256;
-constexpr int kStackLds = 16;
-constexpr int kRefill = 16;
+constexpr int kDefaultLds = 16;
 
-template <int W, bool ANY, bool STATS, int SPILL>
+template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS>
 __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
-    __shared__ uint32_t s_ref[kStackLds * kBlock];
-    __shared__ float s_tm[kStackLds * kBlock];
-    constexpr int CAP = kStackLds + SPILL;
-    uint32_t sp_ref[SPILL > 0 ? SPILL : 1];
-    float sp_tm[SPILL > 0 ? SPILL : 1];
+    __shared__ uint32_t s_ref[LDS * kBlock];
+    __shared__ float s_tm[LDS * kBlock];
+    constexpr int CAP = LDS + SPILL;
+    uint32_t sp_ref[SPILL];
+    float sp_tm[SPILL];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t n_rays = a.n_dev ? *a.n_dev : a.n;
+    const uint32_t refill = a.refill ? a.refill : 16u;
     const NodeRec<W>* __restrict__ nodes = static_cast<const NodeRec<W>*>(a.tree.nodes);
     const PrimRec* __restrict__ prims = a.tree.prims;
+    const QuadRec* __restrict__ quads = a.tree.quads;
 
     RayK r;
     uint32_t ray_id = 0xffffffffu;
@@ -51,12 +56,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
     bool overflow = false;
 
     auto push = [&](int e, uint32_t ref, float t) {
-        if (e < kStackLds) {
+        if (e < LDS) {
             s_ref[e * kBlock + tid] = ref;
             s_tm[e * kBlock + tid] = t;
         } else if (e < CAP) {
-            sp_ref[e - kStackLds] = ref;
-            sp_tm[e - kStackLds] = t;
+            sp_ref[e - LDS] = ref;
+            sp_tm[e - LDS] = t;
         } else {
             overflow = true;
         }
@@ -66,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
         const bool idle = sp == 0;
         const uint64_t want = __ballot(idle && more);
         const uint64_t busy = __ballot(!idle);
-        if (want != 0 && (busy == 0 || __popcll(want) >= kRefill)) {
+        if (want != 0 && (busy == 0 || (uint32_t)__popcll(want) >= refill)) {
             const uint32_t cnt = (uint32_t)__popcll(want);
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)want) - 1;
             uint32_t base = 0;
@@ -84,9 +89,30 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                     bt = FLT_MAX;
                     bp = 0xffffffffu;
                     bu = bv = 0.f;
-                    const float rmin = a.tree.valid ? root_slab(r, a.tree.bbox) : -1.0f;
+                    uint32_t root, valid;
+                    float rmin = -1.0f;
+                    if (MULTI) {
+                        const uint32_t sid = a.ray_scene[my];
+                        if (sid < a.n_trees) {
+                            const TreeView& tv = a.trees[sid];
+                            nodes = static_cast<const NodeRec<W>*>(tv.nodes);
+                            prims = tv.prims;
+                            quads = tv.quads;
+                            root = tv.root;
+                            valid = tv.valid;
+                            if (valid) rmin = root_slab(r, tv.bbox);
+                        } else {
+                            root = 0;
+                            valid = 0;
+                            atomicOr(a.error, 2u);
+                        }
+                    } else {
+                        root = a.tree.root;
+                        valid = a.tree.valid;
+                        if (valid) rmin = root_slab(r, a.tree.bbox);
+                    }
                     if (rmin >= 0.0f) {
-                        push(0, a.tree.root, rmin);
+                        push(0, root, rmin);
                         sp = 1;
                     } else {
                         if (ANY) static_cast<uint8_t*>(a.out)[my] = 0;
@@ -107,11 +133,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
             --sp;
             uint32_t ref;
             float tm;
-            if (sp < kStackLds) {
+            if (sp < LDS) {
                 ref = s_ref[sp * kBlock + tid];
                 tm = s_tm[sp * kBlock + tid];
             } else {
-                const int e = sp - kStackLds < SPILL ? sp - kStackLds : 0;
+                const int e = sp - LDS < SPILL ? sp - LDS : 0;
                 ref = sp_ref[e];
                 tm = sp_tm[e];
             }
@@ -126,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                         bool hit = false;
                         if (pr.flags & PR_QUAD) {
                             if (STATS) n_quads++;
-                            if (quad_test(r, a.tree.quads[pr.aux], bt, t)) {
+                            if (quad_test(r, quads[pr.aux], bt, t)) {
                                 bt = t;
                                 bp = pr.prim;
                                 bu = bv = 0.f;
@@ -206,13 +232,14 @@ __global__ void k_reset(uint32_t* counter) {
     if (threadIdx.x == 0) *counter = 0;
 }
 
-template <int W, bool ANY, bool STATS, int SPILL>
+template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS>
 static hipError_t launch_one(const TraceArgs& a, hipStream_t stream, int num_cus) {
-    auto kern = k_trace<W, ANY, STATS, SPILL>;
+    auto kern = k_trace<W, ANY, STATS, SPILL, MULTI, LDS>;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
+    if (a.blocks_per_cu && (int)a.blocks_per_cu < per_cu) per_cu = (int)a.blocks_per_cu;
     const uint64_t want = ((uint64_t)a.n + kBlock - 1) / kBlock;
     uint64_t grid = (uint64_t)num_cus * (uint64_t)per_cu;
     if (want < grid) grid = want;
@@ -222,25 +249,37 @@ static hipError_t launch_one(const TraceArgs& a, hipStream_t stream, int num_cus
     return hipGetLastError();
 }
 
-template <int W, bool ANY, bool STATS>
-static hipError_t launch_w(const TraceArgs& a, hipStream_t stream, int num_cus) {
-    const uint32_t need = a.tree.max_stack;
-    if (need <= kStackLds) return launch_one<W, ANY, STATS, 0>(a, stream, num_cus);
-    if (need <= kStackLds + 48) return launch_one<W, ANY, STATS, 48>(a, stream, num_cus);
-    if (need <= kStackLds + 240) return launch_one<W, ANY, STATS, 240>(a, stream, num_cus);
+template <int W, bool ANY, bool STATS, bool MULTI, int LDS>
+static hipError_t launch_spill(const TraceArgs& a, uint32_t need, hipStream_t stream, int num_cus) {
+    if (need <= (uint32_t)(LDS + 48)) return launch_one<W, ANY, STATS, 48, MULTI, LDS>(a, stream, num_cus);
+    if (need <= (uint32_t)(LDS + 240)) return launch_one<W, ANY, STATS, 240, MULTI, LDS>(a, stream, num_cus);
     return hipErrorInvalidValue;
+}
+
+template <int W, bool ANY, bool STATS, bool MULTI>
+static hipError_t launch_lds(const TraceArgs& a, uint32_t need, hipStream_t stream, int num_cus) {
+    if (!STATS && a.lds_entries == 8) return launch_spill<W, ANY, STATS, MULTI, 8>(a, need, stream, num_cus);
+    return launch_spill<W, ANY, STATS, MULTI, 16>(a, need, stream, num_cus);
+}
+
+template <int W, bool MULTI>
+static hipError_t launch_w(const TraceArgs& a, bool any, bool stats, hipStream_t stream, int num_cus) {
+    const uint32_t need = MULTI ? a.max_stack : a.tree.max_stack;
+    if (any) return stats ? launch_lds<W, true, true, MULTI>(a, need, stream, num_cus)
+                          : launch_lds<W, true, false, MULTI>(a, need, stream, num_cus);
+    return stats ? launch_lds<W, false, true, MULTI>(a, need, stream, num_cus)
+                 : launch_lds<W, false, false, MULTI>(a, need, stream, num_cus);
 }
 
 hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t stream, int num_cus) {
     if (a.n == 0) return hipSuccess;
-    if (a.tree.width == 8) {
-        if (any) return stats ? launch_w<8, true, true>(a, stream, num_cus) : launch_w<8, true, false>(a, stream, num_cus);
-        return stats ? launch_w<8, false, true>(a, stream, num_cus) : launch_w<8, false, false>(a, stream, num_cus);
-    }
-    if (any) return stats ? launch_w<4, true, true>(a, stream, num_cus) : launch_w<4, true, false>(a, stream, num_cus);
-    return stats ? launch_w<4, false, true>(a, stream, num_cus) : launch_w<4, false, false>(a, stream, num_cus);
+    const bool multi = a.trees != nullptr;
+    const uint32_t width = multi ? a.width : a.tree.width;
+    if (width == 8) return multi ? launch_w<8, true>(a, any, stats, stream, num_cus)
+                                 : launch_w<8, false>(a, any, stats, stream, num_cus);
+    return multi ? launch_w<4, true>(a, any, stats, stream, num_cus) : launch_w<4, false>(a, any, stats, stream, num_cus);
 }
 
-uint32_t trace_max_stack_supported() { return kStackLds + 240; }
+uint32_t trace_max_stack_supported() { return 8 + 240; }
 
 }  // namespace sortgpu

@@ -8,14 +8,24 @@
 namespace sortgpu {
 
 struct TraceArgs {
-    TreeView tree;
+    TreeView tree;                 // single-scene form
     const void* rays;              // n x 32 B ray records (device)
     uint32_t n;                    // ray count, or the capacity when n_dev is set
     const uint32_t* n_dev;         // optional device-side ray count (read at kernel start)
     void* out;                     // closest: n x 16 B hits; any: n x u8 (device)
     uint32_t* counter;             // work counter (device), reset by the launch
-    uint32_t* error;               // error flags (device): bit0 = stack overflow
+    uint32_t* error;               // error flags (device): bit0 stack overflow, bit1 bad scene id
     unsigned long long* stats;     // 4 counters (device), only with stats
+    // multi-scene form: trees != null selects it
+    const TreeView* trees;         // device table of tree views
+    const uint32_t* ray_scene;     // per ray: index into trees (out-of-range ids miss, error bit 2)
+    uint32_t n_trees;
+    uint32_t width;                // node width shared by every tree of the table
+    uint32_t max_stack;            // deepest worst-case stack over the table
+    // tuning (0 = default)
+    uint32_t refill;               // idle lanes that trigger a refill (default 16)
+    uint32_t lds_entries;          // LDS stack entries per lane: 8 or 16 (default 16)
+    uint32_t blocks_per_cu;        // cap on resident workgroups per CU (default: occupancy limit)
 };
 
 // Launches the closest (any = false) or any-hit traversal on `stream`.
