@@ -38,7 +38,7 @@ DEVICE = [
 
 HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp", "pool.cpp"]
 # compiled as HIP: kernels, and host code that shares device record types with them
-DEVICE_SOURCES = ["trace.hip", "pt.hip", "pt_api.cpp"]
+DEVICE_SOURCES = ["trace.hip", "pt.hip", "pt_host.cpp", "pt_api.cpp"]
 
 
 def _compile(src: str) -> str:

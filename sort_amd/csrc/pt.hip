@@ -63,16 +63,29 @@ __device__ __forceinline__ void put_ray(float4* q, uint32_t i, V3 o, float tmin,
 }
 
 // ---- raygen ------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kPtBlock) void k_raygen(const CamDev cam, uint32_t seed, uint32_t npix,
-                                                     uint32_t sample0, uint32_t npaths, PathState ps, Queues q) {
+__device__ __forceinline__ int chunk_of_path(const ChunkTable& ct, uint32_t p) {
+    int k = 0;
+    for (int i = 1; i < (int)ct.n; i++)
+        if (ct.c[i].path_base <= p) k = i;
+    return k;
+}
+
+// Camera rays of every chunk (PerspectiveCamera::GenerateRay in closed form: raster -> camera
+// through the inverse of Scale(W,H,1) Scale(.5,-.5,1) Translate(1,-1,0) Perspective(xs, ys),
+// then the orthonormal ViewLookat basis).
+__global__ __launch_bounds__(kPtBlock) void k_raygen(const ChunkTable ct, const ReqDev* __restrict__ reqs,
+                                                     PathState ps, Queues q) {
     const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
-    if (p == 0) q.count[0] = npaths;
-    if (p >= npaths) return;
-    const uint32_t pix = p % npix;
-    const uint32_t s = sample0 + p / npix;
+    if (p == 0) q.count[0] = ct.total_paths;
+    if (p >= ct.total_paths) return;
+    const ChunkDev& c = ct.c[chunk_of_path(ct, p)];
+    const ReqDev& R = reqs[c.req];
+    const uint32_t local = p - c.path_base;
+    const uint32_t pix = local % c.npix;
+    const uint32_t s = c.sample0 + local / c.npix;
+    const CamDev& cam = R.cam;
     const uint32_t x = pix % cam.width, y = pix / cam.width;
-    const Rng4 j = rng4(seed, pix, s, kCameraBounce, 0);
-    // raster -> camera (inverse of Scale(W,H,1) Scale(.5,-.5,1) Translate(1,-1,0) Perspective(xs, ys))
+    const Rng4 j = rng4(R.seed, pix, s, kCameraBounce, 0);
     const float rx = (float)x + j.a, ry = (float)y + j.b;
     const float cx = 2.0f * rx / (float)cam.width - 1.0f;
     const float cy = 1.0f - 2.0f * ry / (float)cam.height;
@@ -80,29 +93,34 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(const CamDev cam, uint32_t 
     const V3 d = cam.right * dc.x + cam.up * dc.y + cam.fwd * dc.z;
     put_ray(q.ray[0], p, cam.eye, 0.0f, d, FLT_MAX);
     q.pid[0][p] = p;
+    q.rsc[0][p] = R.scene;
     ps.T[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.c0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.c1[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.key[p] = pix;
     ps.smp[p] = s;
+    ps.req[p] = c.req;
 }
 
-// Caller-provided rays (the batched form of Integrator::Li): keys[2i] = pixel, keys[2i+1] = sample.
+// Caller-provided rays (the batched form of Integrator::Li) of request 0: keys[2i] = pixel,
+// keys[2i+1] = sample.
 __global__ __launch_bounds__(kPtBlock) void k_raygen_rays(const float4* rays, const uint32_t* keys, uint32_t n,
-                                                          PathState ps, Queues q) {
+                                                          const ReqDev* __restrict__ reqs, PathState ps, Queues q) {
     const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
     if (p == 0) q.count[0] = n;
     if (p >= n) return;
     q.ray[0][2 * p] = rays[2 * p];
     q.ray[0][2 * p + 1] = rays[2 * p + 1];
     q.pid[0][p] = p;
+    q.rsc[0][p] = reqs[0].scene;
     ps.T[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.c0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.c1[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     ps.key[p] = keys[2 * p];
     ps.smp[p] = keys[2 * p + 1];
+    ps.req[p] = 0;
 }
 
 __global__ void k_zero_counts(uint32_t* count, int which) {
@@ -178,9 +196,9 @@ __device__ __forceinline__ V3 to_world(const Frame& f, V3 v) {
 }
 
 // ---- shade -------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__ scenes, uint32_t seed, uint32_t bounce,
-                                                    uint32_t max_depth, int cur, PathState ps, Queues q) {
-    const SceneDev& sc = scenes[0];
+__global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__ scenes,
+                                                    const ReqDev* __restrict__ reqs, uint32_t bounce, int cur,
+                                                    PathState ps, Queues q) {
     const uint32_t n = q.count[cur];
     const int nxt = cur ^ 1;
     for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
@@ -188,9 +206,13 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__
         bool cont = false, sh0 = false, sh1 = false;
         V3 ro = v3(0, 0, 0), rd = v3(0, 0, 0), so = v3(0, 0, 0), s0d = v3(0, 0, 0), s1d = v3(0, 0, 0);
         float s0max = 0.f, s1max = 0.f;
-        uint32_t p = 0;
+        uint32_t p = 0, scene = 0;
         if (i < n) {
             p = q.pid[cur][i];
+            const ReqDev& R = reqs[ps.req[p]];
+            scene = R.scene;
+            const SceneDev& sc = scenes[scene];
+            const uint32_t seed = R.seed, max_depth = R.max_depth;
             const float4 h = q.hit[i];
             const float4 r0 = q.ray[cur][2 * i], r1 = q.ray[cur][2 * i + 1];
             const V3 o = v3(r0.x, r0.y, r0.z), d = v3(r1.x, r1.y, r1.z);
@@ -322,16 +344,19 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__
         if (cont) {
             put_ray(q.ray[nxt], ci, ro, 0.0001f, rd, FLT_MAX);
             q.pid[nxt][ci] = p;
+            q.rsc[nxt][ci] = scene;
         }
         const uint32_t si0 = wave_append(sh0, &q.count[2]);
         if (sh0) {
             put_ray(q.sray, si0, so, 0.01f, s0d, s0max);
             q.ssid[si0] = 2 * p;
+            q.ssc[si0] = scene;
         }
         const uint32_t si1 = wave_append(sh1, &q.count[2]);
         if (sh1) {
             put_ray(q.sray, si1, so, 0.001f, s1d, s1max);
             q.ssid[si1] = 2 * p + 1;
+            q.ssc[si1] = scene;
         }
     }
 }
@@ -349,23 +374,31 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve(PathState ps, Queues q) {
 }
 
 // ---- finalize ----------------------------------------------------------------------------------
-// Pixel i owns paths i, i + npix, ... (one per sample of the batch, in sample order).
-__global__ __launch_bounds__(kPtBlock) void k_finalize(PathState ps, uint32_t npix, uint32_t nsamp, float4* film) {
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    if (i >= npix) return;
-    float4 acc = film[i];
-    for (uint32_t s = 0; s < nsamp; s++) {
-        const uint32_t p = s * npix + i;
+// One thread per (chunk, pixel): adds the chunk's samples of that pixel, in sample order, to the
+// request's film (a request has at most one chunk per wave, so no two threads share a pixel).
+__global__ __launch_bounds__(kPtBlock) void k_finalize(const ChunkTable ct, const ReqDev* __restrict__ reqs,
+                                                       PathState ps) {
+    const uint32_t g = blockIdx.x * kPtBlock + threadIdx.x;
+    if (g >= ct.total_pix) return;
+    int k = 0;
+    for (int i = 1; i < (int)ct.n; i++)
+        if (ct.c[i].pix_base <= g) k = i;
+    const ChunkDev& c = ct.c[k];
+    const uint32_t pix = g - c.pix_base;
+    float4* film = reqs[c.req].film;
+    float4 acc = film[pix];
+    for (uint32_t s = 0; s < c.nsamp; s++) {
+        const uint32_t p = c.path_base + s * c.npix + pix;
         const float4 l = ps.L[p], a = ps.c0[p], b = ps.c1[p];
-        const float r = l.x + a.x + b.x, g = l.y + a.y + b.y, bb = l.z + a.z + b.z;
-        if (isfinite(r) && isfinite(g) && isfinite(bb)) {
+        const float r = l.x + a.x + b.x, gg = l.y + a.y + b.y, bb = l.z + a.z + b.z;
+        if (isfinite(r) && isfinite(gg) && isfinite(bb)) {
             acc.x += r;
-            acc.y += g;
+            acc.y += gg;
             acc.z += bb;
             acc.w += 1.0f;
         }
     }
-    film[i] = acc;
+    film[pix] = acc;
 }
 
 // Per-ray radiance for the caller-ray form.
@@ -404,7 +437,7 @@ hipError_t pt_zero_f4(float4* a, uint32_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Runs the bounce loop for the paths already in queue 0 (count q.count[0] = npaths).
+// Runs the bounce loop for the paths already in queue 0 (count q.count[0] <= npaths).
 hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
     const unsigned shade_blocks = std::min<unsigned>(blocks_for(npaths), (unsigned)L.num_cus * 8u);
     hipError_t e;
@@ -415,16 +448,23 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
         ta.n = npaths;
         ta.n_dev = &L.q.count[cur];
         ta.out = L.q.hit;
+        if (L.trees) {
+            ta.trees = L.trees;
+            ta.ray_scene = L.q.rsc[cur];
+            ta.n_trees = L.n_trees;
+            ta.width = L.tree_width;
+            ta.max_stack = L.tree_max_stack;
+        }
         if ((e = launch_trace(ta, false, false, s, L.num_cus)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, L.q.count, nxt);
-        hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.seed, b, L.max_depth, cur,
-                           L.ps, L.q);
+        hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.reqs, b, cur, L.ps, L.q);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        TraceArgs sa = L.trace;
+        TraceArgs sa = ta;
         sa.rays = L.q.sray;
         sa.n = 2 * npaths;
         sa.n_dev = &L.q.count[2];
         sa.out = L.q.socc;
+        if (L.trees) sa.ray_scene = L.q.ssc;
         if ((e = launch_trace(sa, true, false, s, L.num_cus)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_resolve, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.ps, L.q);
         hipLaunchKernelGGL(k_add_totals, dim3(1), dim3(64), 0, s, L.q.count, cur, L.q.totals);
@@ -433,16 +473,13 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t pt_render_batch(const PtLaunch& L, const CamDev& cam, uint32_t sample0, uint32_t nsamp, float4* film,
-                           hipStream_t s) {
-    const uint32_t npix = cam.width * cam.height;
-    const uint32_t npaths = npix * nsamp;
-    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(npaths)), dim3(kPtBlock), 0, s, cam, L.seed, npix, sample0, npaths,
-                       L.ps, L.q);
+hipError_t pt_wave(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
+    if (ct.total_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(ct.total_paths)), dim3(kPtBlock), 0, s, ct, L.reqs, L.ps, L.q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if ((e = pt_bounces(L, npaths, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(npix)), dim3(kPtBlock), 0, s, L.ps, npix, nsamp, film);
+    if ((e = pt_bounces(L, ct.total_paths, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(ct.total_pix)), dim3(kPtBlock), 0, s, ct, L.reqs, L.ps);
     return hipGetLastError();
 }
 
@@ -452,7 +489,7 @@ hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStr
 }
 
 hipError_t pt_li_rays(const PtLaunch& L, const float4* rays, const uint32_t* keys, uint32_t n, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_raygen_rays, dim3(blocks_for(n)), dim3(kPtBlock), 0, s, rays, keys, n, L.ps, L.q);
+    hipLaunchKernelGGL(k_raygen_rays, dim3(blocks_for(n)), dim3(kPtBlock), 0, s, rays, keys, n, L.reqs, L.ps, L.q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if ((e = pt_bounces(L, n, s)) != hipSuccess) return e;

@@ -1,0 +1,297 @@
+// Host-side helpers of the path tracer: device allocation lists, the path workspace, scene
+// (shading data) upload and the camera basis.
+#include "pt_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace sortgpu {
+
+int DeviceAllocs::alloc(sortgpu_ctx* ctx, void** p, size_t n, const char* what) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, std::max<size_t>(n, 16));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e == hipErrorOutOfMemory ? (set_error(ctx, "%s: out of device memory", what), SORTGPU_ERR_OOM)
+                                        : hip_fail(ctx, e, what);
+    }
+    ptrs.push_back(*p);
+    bytes += n;
+    return SORTGPU_OK;
+}
+
+int DeviceAllocs::upload(sortgpu_ctx* ctx, const void* src, size_t n, void** dst, const char* what) {
+    *dst = nullptr;
+    if (!src || !n) return SORTGPU_OK;
+    int rc = alloc(ctx, dst, n, what);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(*dst, src, n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(ctx, e, what);
+    return SORTGPU_OK;
+}
+
+void DeviceAllocs::free_all() {
+    for (void* p : ptrs) hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+}
+
+namespace {
+
+template <class T>
+int ws(DeviceAllocs& m, sortgpu_ctx* ctx, T** out, size_t count) {
+    void* p = nullptr;
+    int rc = m.alloc(ctx, &p, count * sizeof(T), "hipMalloc(path workspace)");
+    *out = static_cast<T*>(p);
+    return rc;
+}
+
+template <class T>
+int up(DeviceAllocs& m, sortgpu_ctx* ctx, const T* src, size_t count, const T** dst, const char* what) {
+    void* p = nullptr;
+    int rc = m.upload(ctx, src, count * sizeof(T), &p, what);
+    *dst = static_cast<const T*>(p);
+    return rc;
+}
+
+float lum(const float* c) { return 0.212671f * c[0] + 0.715160f * c[1] + 0.072169f * c[2]; }
+
+}  // namespace
+
+size_t PathWorkspace::bytes_for(uint32_t paths) { return (size_t)std::max(paths, 1u) * kPathBytes + 16 * 4 + 32; }
+
+int PathWorkspace::ensure(sortgpu_ctx* ctx, uint32_t paths) {
+    if (paths <= cap) return SORTGPU_OK;
+    release();
+    paths = std::max(paths, 1u);
+    const size_t n = paths;
+    int rc;
+    if ((rc = ws(mem, ctx, &ps.T, n)) || (rc = ws(mem, ctx, &ps.L, n)) || (rc = ws(mem, ctx, &ps.c0, n)) ||
+        (rc = ws(mem, ctx, &ps.c1, n)) || (rc = ws(mem, ctx, &ps.key, n)) || (rc = ws(mem, ctx, &ps.smp, n)) ||
+        (rc = ws(mem, ctx, &ps.req, n)) || (rc = ws(mem, ctx, &q.ray[0], 2 * n)) ||
+        (rc = ws(mem, ctx, &q.ray[1], 2 * n)) || (rc = ws(mem, ctx, &q.pid[0], n)) ||
+        (rc = ws(mem, ctx, &q.pid[1], n)) || (rc = ws(mem, ctx, &q.rsc[0], n)) || (rc = ws(mem, ctx, &q.rsc[1], n)) ||
+        (rc = ws(mem, ctx, &q.hit, n)) || (rc = ws(mem, ctx, &q.sray, 4 * n)) || (rc = ws(mem, ctx, &q.ssid, 2 * n)) ||
+        (rc = ws(mem, ctx, &q.ssc, 2 * n)) || (rc = ws(mem, ctx, &q.socc, 2 * n)) ||
+        (rc = ws(mem, ctx, &q.count, 16)) || (rc = ws(mem, ctx, &q.totals, 4))) {
+        release();
+        return rc;
+    }
+    cap = paths;
+    return SORTGPU_OK;
+}
+
+void PathWorkspace::release() {
+    mem.free_all();
+    ps = PathState{};
+    q = Queues{};
+    cap = 0;
+}
+
+int ReqRing::init(sortgpu_ctx* ctx) {
+    const size_t n = (size_t)kSlots * kMaxChunks * sizeof(ReqDev);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host), n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        host = nullptr;
+        return hip_fail(ctx, e, "hipHostMalloc(request ring)");
+    }
+    if ((e = hipMalloc(reinterpret_cast<void**>(&dev), n)) != hipSuccess) {
+        dev = nullptr;
+        release();
+        return hip_fail(ctx, e, "hipMalloc(request ring)");
+    }
+    for (int i = 0; i < kSlots; i++)
+        if ((e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming)) != hipSuccess) {
+            ev[i] = nullptr;
+            release();
+            return hip_fail(ctx, e, "hipEventCreate(request ring)");
+        }
+    return SORTGPU_OK;
+}
+
+void ReqRing::release() {
+    for (int i = 0; i < kSlots; i++)
+        if (ev[i]) {
+            hipEventSynchronize(ev[i]);
+            hipEventDestroy(ev[i]);
+            ev[i] = nullptr;
+        }
+    if (host) hipHostFree(host);
+    if (dev) hipFree(dev);
+    host = nullptr;
+    dev = nullptr;
+}
+
+int ReqRing::push(sortgpu_ctx* ctx, const ReqDev* r, uint32_t n, hipStream_t s, const ReqDev** d_table) {
+    if (n == 0 || n > (uint32_t)kMaxChunks) {
+        set_error(ctx, "request table: %u requests (1..%d)", n, kMaxChunks);
+        return SORTGPU_ERR_ARG;
+    }
+    const uint32_t slot = next;
+    next = (next + 1) % kSlots;
+    hipError_t e = hipEventSynchronize(ev[slot]);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventSynchronize(request ring)");
+    ReqDev* h = host + (size_t)slot * kMaxChunks;
+    ReqDev* d = dev + (size_t)slot * kMaxChunks;
+    std::memcpy(h, r, n * sizeof(ReqDev));
+    if ((e = hipMemcpyAsync(d, h, n * sizeof(ReqDev), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(ctx, e, "hipMemcpyAsync(request table)");
+    last = slot;
+    *d_table = d;
+    return SORTGPU_OK;
+}
+
+int ReqRing::done(sortgpu_ctx* ctx, hipStream_t s) {
+    hipError_t e = hipEventRecord(ev[last], s);
+    return e == hipSuccess ? SORTGPU_OK : hip_fail(ctx, e, "hipEventRecord(request ring)");
+}
+
+int upload_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scene_desc* desc, SceneDev& h,
+                 DeviceAllocs& mem) {
+    const uint32_t ntris = (uint32_t)(acc->idx.size() / 3), nverts = (uint32_t)(acc->pos.size() / 3);
+    const uint32_t nquads = (uint32_t)acc->quads.size();
+    if (desc->nmaterials && !desc->materials) {
+        set_error(ctx, "scene: materials is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    if (nquads && !desc->quad_radiance) {
+        set_error(ctx, "scene: quad_radiance is NULL but the accelerator has quads");
+        return SORTGPU_ERR_ARG;
+    }
+    if (desc->tri_material) {
+        for (uint32_t i = 0; i < ntris; i++)
+            if (desc->tri_material[i] >= std::max(desc->nmaterials, 1u)) {
+                set_error(ctx, "scene: triangle %u has material %u (of %u)", i, desc->tri_material[i],
+                          desc->nmaterials);
+                return SORTGPU_ERR_ARG;
+            }
+    } else if (ntris && desc->nmaterials == 0) {
+        set_error(ctx, "scene: triangles need at least one material");
+        return SORTGPU_ERR_ARG;
+    }
+    for (uint32_t m = 0; m < desc->nmaterials; m++)
+        if (desc->materials[m].type > SORTGPU_MAT_DISNEY) {
+            set_error(ctx, "scene: material %u has unknown type %u", m, desc->materials[m].type);
+            return SORTGPU_ERR_ARG;
+        }
+
+    // materials + the default white Lambert
+    std::vector<MatRec> mats(desc->nmaterials + 1);
+    std::memset(mats.data(), 0, mats.size() * sizeof(MatRec));
+    for (uint32_t m = 0; m < desc->nmaterials; m++) {
+        const sortgpu_material& s = desc->materials[m];
+        MatRec& d = mats[m];
+        d.type = s.type;
+        d.roughness = s.roughness;
+        d.ior = s.ior > 0.0f ? s.ior : 1.5f;
+        d.base = v3(s.base[0], s.base[1], s.base[2]);
+        d.trans = v3(s.trans[0], s.trans[1], s.trans[2]);
+        d.eta = v3(s.eta[0], s.eta[1], s.eta[2]);
+        d.k = v3(s.k[0], s.k[1], s.k[2]);
+        d.metallic = s.metallic;
+        d.specular = s.specular;
+        d.specular_tint = s.specular_tint;
+        d.sheen = s.sheen;
+        d.sheen_tint = s.sheen_tint;
+        d.clearcoat = s.clearcoat;
+        d.clearcoat_gloss = s.clearcoat_gloss;
+    }
+    mats.back().type = MAT_LAMBERT;
+    mats.back().base = v3(1.f, 1.f, 1.f);
+
+    // lights: area quads in quad order, then the environment (Scene::genLightDistribution)
+    std::vector<LightRec> lights;
+    std::vector<float> power;
+    std::vector<int32_t> quad_light(nquads);
+    for (uint32_t qi = 0; qi < nquads; qi++) {
+        const sortgpu_quad& q = acc->quads[qi];
+        LightRec l;
+        std::memset(&l, 0, sizeof(l));
+        std::memcpy(l.mat, q.mat, 12 * sizeof(float));
+        std::memcpy(l.inv, q.inv, 16 * sizeof(float));
+        const float* Lq = desc->quad_radiance + 3 * qi;
+        l.L = v3(Lq[0], Lq[1], Lq[2]);
+        l.area = q.size_x * q.size_y;
+        l.half_x = q.size_x * 0.5f;
+        l.half_y = q.size_y * 0.5f;
+        l.prim = ntris + qi;
+        l.kind = 0;
+        quad_light[qi] = (int32_t)lights.size();
+        lights.push_back(l);
+        power.push_back(l.area * lum(Lq) * 6.28318530717958647692f);  // AreaLight::Power
+    }
+    int32_t env_idx = -1;
+    const float* E = desc->env_radiance;
+    if (E[0] != 0.0f || E[1] != 0.0f || E[2] != 0.0f) {
+        LightRec l;
+        std::memset(&l, 0, sizeof(l));
+        l.L = v3(E[0], E[1], E[2]);
+        l.kind = 1;
+        env_idx = (int32_t)lights.size();
+        lights.push_back(l);
+        const float* b = acc->host.bbox;
+        const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+        const float r = std::sqrt(dx * dx + dy * dy + dz * dz) * 0.5f;
+        power.push_back(r * r * 3.14159265358979323846f * lum(E));  // SkyLight::Power
+    }
+    const uint32_t nl = (uint32_t)lights.size();
+    std::vector<float> cdf(nl + 1, 0.0f);  // Distribution1D (core/samplemethod.h:132-152)
+    for (uint32_t i = 0; i < nl; i++) cdf[i + 1] = cdf[i] + power[i];
+    const float sum = nl ? cdf[nl] : 0.0f;
+    for (uint32_t i = 0; i <= nl && nl; i++) cdf[i] = sum != 0.0f ? cdf[i] / sum : (float)i / (float)nl;
+
+    std::memset(&h, 0, sizeof(h));
+    h.tree = acc->view;
+    h.ntris = ntris;
+    h.nquads = nquads;
+    h.nlights = nl;
+    h.nmats = desc->nmaterials;
+    h.env_light = env_idx;
+    h.env_L = v3(E[0], E[1], E[2]);
+    int rc;
+    if ((rc = up(mem, ctx, acc->idx.data(), acc->idx.size(), &h.idx, "upload(idx)")) ||
+        (rc = up(mem, ctx, acc->pos.data(), acc->pos.size(), &h.pos, "upload(pos)")) ||
+        (rc = up(mem, ctx, desc->normals, desc->normals ? 3 * (size_t)nverts : 0, &h.nrm, "upload(normals)")) ||
+        (rc = up(mem, ctx, desc->tangents, desc->tangents ? 3 * (size_t)nverts : 0, &h.tan, "upload(tangents)")) ||
+        (rc = up(mem, ctx, desc->tri_material, desc->tri_material ? (size_t)ntris : 0, &h.tri_mat,
+                 "upload(materials)")) ||
+        (rc = up(mem, ctx, mats.data(), mats.size(), &h.mats, "upload(material table)")) ||
+        (rc = up(mem, ctx, lights.data(), lights.size(), &h.lights, "upload(lights)")) ||
+        (rc = up(mem, ctx, cdf.data(), cdf.size(), &h.light_cdf, "upload(light cdf)")) ||
+        (rc = up(mem, ctx, quad_light.data(), quad_light.size(), &h.quad_light, "upload(quad lights)")))
+        return rc;
+    return SORTGPU_OK;
+}
+
+int make_camera(sortgpu_ctx* ctx, const sortgpu_camera* cam, uint32_t width, uint32_t height, CamDev& cd) {
+    const V3 eye = v3(cam->eye[0], cam->eye[1], cam->eye[2]);
+    V3 f = v3(cam->target[0] - eye.x, cam->target[1] - eye.y, cam->target[2] - eye.z);
+    const float fl = std::sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
+    if (!(fl > 0.0f) || !(cam->fov_y > 0.0f) || !(cam->fov_y < 3.14159f)) {
+        set_error(ctx, "camera: degenerate (eye == target or fov outside (0, pi))");
+        return SORTGPU_ERR_ARG;
+    }
+    f = f / fl;
+    const V3 up = v3(cam->up[0], cam->up[1], cam->up[2]);
+    V3 r = v3(up.y * f.z - up.z * f.y, up.z * f.x - up.x * f.z, up.x * f.y - up.y * f.x);  // cross(up, dir)
+    const float rl = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+    if (!(rl > 0.0f)) {
+        set_error(ctx, "camera: up is parallel to the view direction");
+        return SORTGPU_ERR_ARG;
+    }
+    r = r / rl;
+    const V3 u2 = v3(f.y * r.z - f.z * r.y, f.z * r.x - f.x * r.z, f.x * r.y - f.y * r.x);  // cross(dir, right)
+    cd.eye = eye;
+    cd.right = r;
+    cd.up = u2;
+    cd.fwd = f;
+    const float ys = 1.0f / std::tan(cam->fov_y * 0.5f);
+    cd.y_scale = ys;
+    cd.x_scale = ys / ((float)width / (float)height);
+    cd.width = width;
+    cd.height = height;
+    return SORTGPU_OK;
+}
+
+}  // namespace sortgpu

@@ -9,19 +9,26 @@
 namespace sortgpu {
 
 struct PtLaunch {
-    TraceArgs trace;            // tree, counter and error slots; rays / n / out set per stage
-    const SceneDev* scenes;     // device pointer (one scene)
+    TraceArgs trace;            // counter / error slots and tuning; the single tree when trees == null
+    const SceneDev* scenes;     // device scene table
+    const ReqDev* reqs;         // device request table
+    const TreeView* trees;      // device tree table for multi-scene waves, or null
+    uint32_t n_trees;
+    uint32_t tree_width;        // node width of the tree table
+    uint32_t tree_max_stack;    // deepest stack over the tree table
     PathState ps;
     Queues q;
-    uint32_t seed;
-    uint32_t max_depth;
+    uint32_t max_depth;         // deepest request of the wave
     int num_cus;
 };
 
+// Bytes of path workspace per path slot (state + queues), used for HBM budgeting.
+constexpr size_t kPathBytes = 4 * 16 + 3 * 4 + 2 * (32 + 4 + 4) + 16 + 2 * 32 + 2 * 4 + 2 * 4 + 2;
+
 hipError_t pt_zero_f4(float4* a, uint32_t n, hipStream_t s);
 hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s);
-hipError_t pt_render_batch(const PtLaunch& L, const CamDev& cam, uint32_t sample0, uint32_t nsamp, float4* film,
-                           hipStream_t s);
+// raygen + bounce loop + finalize for the chunks of one wave.
+hipError_t pt_wave(const PtLaunch& L, const ChunkTable& ct, hipStream_t s);
 hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStream_t s);
 hipError_t pt_li_rays(const PtLaunch& L, const float4* rays, const uint32_t* keys, uint32_t n, float* out,
                       hipStream_t s);

@@ -1,4 +1,5 @@
-// Path-tracer data shared by the host driver and the kernels.
+// Path-tracer data shared by the host drivers (single render, batched Li, served waves) and the
+// kernels.
 #pragma once
 #include <stdint.h>
 
@@ -30,26 +31,49 @@ struct CamDev {
     uint32_t width, height;
 };
 
-// Path slots of one batch (fixed for the batch; index = path id).
+// One render request: which scene, how to shoot primary rays, where its samples accumulate.
+struct ReqDev {
+    uint32_t scene;      // index into the scene (and tree) tables
+    uint32_t seed;
+    uint32_t max_depth;
+    uint32_t npix;
+    CamDev cam;
+    float4* film;        // npix accumulators: summed rgb of valid samples, valid count
+};
+
+// A wave's work: chunks of whole samples of requests. Paths [path_base, path_base + npix*nsamp)
+// belong to chunk k (pixel-major within a sample); pixels [pix_base, pix_base + npix) of the
+// finalize grid. Passed by value as a kernel argument.
+constexpr int kMaxChunks = 32;
+struct ChunkDev {
+    uint32_t req, sample0, nsamp, npix, path_base, pix_base;
+};
+struct ChunkTable {
+    uint32_t n, total_paths, total_pix, pad;
+    ChunkDev c[kMaxChunks];
+};
+
+// Path slots of one wave (fixed for the wave; index = path id).
 struct PathState {
     float4* T;        // throughput
     float4* L;        // radiance so far
     float4* c0;       // pending light-sampled NEE contribution of the last vertex
     float4* c1;       // pending BSDF-sampled NEE contribution of the last vertex
-    uint32_t* key;    // per path: scene-local pixel index (RNG key)
+    uint32_t* key;    // per path: pixel index (RNG key)
     uint32_t* smp;    // per path: sample index (RNG counter)
-    uint32_t* req;    // per path: request id (served form), 0 otherwise
+    uint32_t* req;    // per path: request index
 };
 
 struct Queues {
     float4* ray[2];     // 2 x float4 per entry
     uint32_t* pid[2];
+    uint32_t* rsc[2];   // scene of each extension entry (multi-scene traversal)
     float4* hit;        // per entry of the current queue
     float4* sray;       // shadow rays
     uint32_t* ssid;     // shadow slot: 2 * path + k
+    uint32_t* ssc;      // scene of each shadow entry
     uint8_t* socc;      // shadow occlusion
-    uint32_t* count;    // [0],[1] extension queue counts, [2] shadow count, [3] spare,
-                        // [4] total extension rays, [5] total shadow rays (64-bit pairs below)
+    uint32_t* count;    // [0],[1] extension queue counts, [2] shadow count
     unsigned long long* totals;  // [0] extension rays, [1] shadow rays
 };
 
