@@ -214,8 +214,88 @@ int sortgpu_li(sortgpu_scene* scene, const sortgpu_ray* d_rays, const uint32_t* 
                uint32_t max_depth, uint64_t seed, float* d_rgb, void* stream);
 
 /* Totals of the last render / li call (synchronous): out[0] extension rays, out[1] shadow
- * rays, out[2] paths, out[3] microseconds of the last call (host wall clock to completion). */
+ * rays, out[2] paths, out[3] microseconds of the last call on its stream (event timing). */
 int sortgpu_render_stats(sortgpu_scene* scene, uint64_t out[4]);
+
+/* ---- Served form: continuous-batching render server ------------------------------------------
+ * No counterpart in the reference, which renders one job per process (sort.cpp:32-60,
+ * core/image_evaluation.cpp). The server keeps scenes, path state and films resident in HBM,
+ * admits requests by an HBM budget with least-recently-used scene eviction and interleaves
+ * samples of many requests in each wave of the bounce loop. Every request's image equals, bit for
+ * bit, what sortgpu_render gives for the same scene, camera, size, spp, depth and seed. */
+typedef struct sortgpu_server sortgpu_server;
+
+typedef struct sortgpu_server_opts {
+    uint64_t hbm_budget;     /* bytes the server may hold: scenes + films + workspace (0 = 96 GiB) */
+    uint32_t wave_paths;     /* path slots per wave (0 = 4,194,304) */
+    uint32_t max_active;     /* requests admitted at once, 1..32 (0 = 32) */
+    uint32_t inflight_waves; /* waves in flight at once, 1..4 (0 = 2), each on a stream and path
+                                workspace of its own, so their bounce loops overlap */
+    uint32_t manual;         /* 1: no worker thread; the caller drives sortgpu_server_step */
+    uint32_t host_device;    /* 1: host-only stand-in device (no GPU; films hold a hash of the
+                                path key) for tests of the scheduling and failure handling */
+    uint32_t fail_alloc_at;  /* test hook: the n-th budgeted allocation (scene load or film)
+                                fails as if out of memory (0 = never) */
+    uint32_t reserved[2];
+} sortgpu_server_opts;
+
+typedef struct sortgpu_request {
+    uint32_t scene;          /* id from sortgpu_server_add_scene */
+    uint32_t width, height;  /* at most wave_paths pixels */
+    uint32_t spp;
+    uint32_t max_depth;
+    uint32_t reserved;
+    uint64_t seed;
+    sortgpu_camera camera;
+    double timeout_s;        /* > 0: fails with SORTGPU_REQ_TIMED_OUT if not done this long after submission */
+} sortgpu_request;
+
+enum {
+    SORTGPU_REQ_QUEUED = 0,
+    SORTGPU_REQ_ACTIVE = 1,
+    SORTGPU_REQ_DONE = 2,
+    SORTGPU_REQ_FAILED = 3,
+    SORTGPU_REQ_CANCELLED = 4,
+    SORTGPU_REQ_TIMED_OUT = 5
+};
+
+typedef struct sortgpu_request_info {
+    int32_t state;           /* SORTGPU_REQ_* */
+    int32_t status;          /* SORTGPU_OK, or the failure's status code */
+    uint32_t samples_done;
+    uint32_t spp;
+    double t_submit, t_admit, t_done;  /* seconds since the server started; 0 = not yet */
+} sortgpu_request_info;
+
+int sortgpu_server_create(sortgpu_ctx* ctx, const sortgpu_server_opts* opts, sortgpu_server** out);
+/* Cancels what is outstanding, waits for the waves in flight and frees everything. */
+void sortgpu_server_destroy(sortgpu_server* srv);
+/* Registers a scene: the accelerator's tree (8- or 4-wide; one width per server) and the shading
+ * data are copied to the host side of the server, so both may be destroyed afterwards. The scene
+ * is uploaded when a request needs it and may be evicted and reloaded later. */
+int sortgpu_server_add_scene(sortgpu_server* srv, const sortgpu_accel* accel, const sortgpu_scene_desc* desc,
+                             uint32_t* scene_id);
+/* Queues a request; invalid parameters are refused here with SORTGPU_ERR_ARG. Thread-safe. */
+int sortgpu_server_submit(sortgpu_server* srv, const sortgpu_request* req, uint64_t* id);
+int sortgpu_server_cancel(sortgpu_server* srv, uint64_t id);
+int sortgpu_server_info(sortgpu_server* srv, uint64_t id, sortgpu_request_info* out);
+/* Waits up to timeout_s (< 0: forever) for the request to finish; fills *out either way. */
+int sortgpu_server_wait(sortgpu_server* srv, uint64_t id, double timeout_s, sortgpu_request_info* out);
+/* Copies a finished request's image (width * height * 3 floats, host memory). */
+int sortgpu_server_result(sortgpu_server* srv, uint64_t id, float* rgb, uint64_t n_floats);
+/* Copies the failure message of a request (empty when none) into buf (NUL-terminated). */
+int sortgpu_server_message(sortgpu_server* srv, uint64_t id, char* buf, uint32_t cap);
+/* Forgets a request (cancelling it when outstanding); its id becomes unknown. */
+int sortgpu_server_release(sortgpu_server* srv, uint64_t id);
+/* Manual mode: one scheduling pass (admit, launch, retire; blocks on the oldest wave in flight).
+ * *busy = 1 while requests are queued, active or in flight. */
+int sortgpu_server_step(sortgpu_server* srv, int* busy);
+/* Waits until nothing is queued, active or in flight (manual mode: steps until then). */
+int sortgpu_server_drain(sortgpu_server* srv, double timeout_s);
+/* out[0] waves, [1] paths, [2] extension rays, [3] shadow rays, [4] scene loads, [5] evictions,
+ * [6] resident scenes, [7] budgeted bytes in use, [8] peak budgeted bytes, [9] done, [10] failed,
+ * [11] cancelled, [12] timed out, [13] device busy microseconds, [14] budget, [15] workspace bytes. */
+int sortgpu_server_stats(sortgpu_server* srv, uint64_t out[16]);
 
 #ifdef __cplusplus
 }

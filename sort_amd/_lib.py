@@ -61,6 +61,24 @@ class Camera(ctypes.Structure):
                 ("fov_y", ctypes.c_float)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("wave_paths", ctypes.c_uint32), ("max_active", ctypes.c_uint32),
+                ("inflight_waves", ctypes.c_uint32), ("manual", ctypes.c_uint32), ("host_device", ctypes.c_uint32),
+                ("fail_alloc_at", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64), ("camera", Camera), ("timeout_s", ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("samples_done", ctypes.c_uint32),
+                ("spp", ctypes.c_uint32), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -91,6 +109,19 @@ _SIGS = {
     "sortgpu_render": (_I, [_P, ctypes.POINTER(Camera), _U32, _U32, _U32, _U32, ctypes.c_uint64, _P, _P]),
     "sortgpu_li": (_I, [_P, _P, _P, _U32, _U32, ctypes.c_uint64, _P, _P]),
     "sortgpu_render_stats": (_I, [_P, _P]),
+    "sortgpu_server_create": (_I, [_P, ctypes.POINTER(ServerOpts), ctypes.POINTER(_P)]),
+    "sortgpu_server_destroy": (None, [_P]),
+    "sortgpu_server_add_scene": (_I, [_P, _P, ctypes.POINTER(SceneDesc), ctypes.POINTER(_U32)]),
+    "sortgpu_server_submit": (_I, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    "sortgpu_server_cancel": (_I, [_P, ctypes.c_uint64]),
+    "sortgpu_server_info": (_I, [_P, ctypes.c_uint64, ctypes.POINTER(RequestInfo)]),
+    "sortgpu_server_wait": (_I, [_P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(RequestInfo)]),
+    "sortgpu_server_result": (_I, [_P, ctypes.c_uint64, _P, ctypes.c_uint64]),
+    "sortgpu_server_message": (_I, [_P, ctypes.c_uint64, ctypes.c_char_p, _U32]),
+    "sortgpu_server_release": (_I, [_P, ctypes.c_uint64]),
+    "sortgpu_server_step": (_I, [_P, ctypes.POINTER(_I)]),
+    "sortgpu_server_drain": (_I, [_P, ctypes.c_double]),
+    "sortgpu_server_stats": (_I, [_P, _P]),
 }
 
 
@@ -105,6 +136,13 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        # torch ships its own HIP runtime under the same soname as the system one; whichever loads
+        # first serves both. Loading torch first keeps the process on the runtime torch was built
+        # for (torch fails to see the GPU when it finds the other one already loaded).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(l, name)

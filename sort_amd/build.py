@@ -36,9 +36,9 @@ DEVICE = [
     "-munsafe-fp-atomics",
 ]
 
-HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp", "pool.cpp"]
+HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp", "pool.cpp", "serve_engine.cpp", "serve_api.cpp"]
 # compiled as HIP: kernels, and host code that shares device record types with them
-DEVICE_SOURCES = ["trace.hip", "pt.hip", "pt_host.cpp", "pt_api.cpp"]
+DEVICE_SOURCES = ["trace.hip", "pt.hip", "pt_host.cpp", "pt_api.cpp", "serve_device.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -473,14 +473,23 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t pt_wave(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
+hipError_t pt_wave_paths(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
     if (ct.total_paths == 0) return hipSuccess;
     hipLaunchKernelGGL(k_raygen, dim3(blocks_for(ct.total_paths)), dim3(kPtBlock), 0, s, ct, L.reqs, L.ps, L.q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if ((e = pt_bounces(L, ct.total_paths, s)) != hipSuccess) return e;
+    return pt_bounces(L, ct.total_paths, s);
+}
+
+hipError_t pt_wave_film(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
+    if (ct.total_pix == 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(ct.total_pix)), dim3(kPtBlock), 0, s, ct, L.reqs, L.ps);
     return hipGetLastError();
+}
+
+hipError_t pt_wave(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
+    hipError_t e = pt_wave_paths(L, ct, s);
+    return e != hipSuccess ? e : pt_wave_film(L, ct, s);
 }
 
 hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStream_t s) {

@@ -88,9 +88,22 @@ int sortgpu_scene_create(sortgpu_accel* acc, const sortgpu_scene_desc* desc, sor
     sortgpu_scene* sc = new (std::nothrow) sortgpu_scene();
     if (!sc) return SORTGPU_ERR_OOM;
     sc->accel = acc;
+    SceneImage im;
+    void* block = nullptr;
     void* dsc = nullptr;
-    if ((rc = upload_scene(ctx, acc, desc, sc->host, sc->mem)) ||
-        (rc = sc->mem.upload(ctx, &sc->host, sizeof(SceneDev), &dsc, "upload(scene)")) || (rc = sc->ring.init(ctx))) {
+    if ((rc = prepare_scene(ctx, acc, desc, false, im)) ||
+        (rc = sc->mem.alloc(ctx, &block, im.block_bytes(), "hipMalloc(scene)")) ||
+        (rc = im.place(ctx, block, sc->host, nullptr))) {
+        sortgpu_scene_destroy(sc);
+        return rc;
+    }
+    if (hipStreamSynchronize(nullptr) != hipSuccess) {  // `im` goes out of scope
+        set_error(ctx, "sortgpu_scene_create: scene upload failed");
+        sortgpu_scene_destroy(sc);
+        return SORTGPU_ERR_HIP;
+    }
+    sc->host.tree = acc->view;
+    if ((rc = sc->mem.upload(ctx, &sc->host, sizeof(SceneDev), &dsc, "upload(scene)")) || (rc = sc->ring.init(ctx))) {
         sortgpu_scene_destroy(sc);
         return rc;
     }

@@ -1,5 +1,5 @@
-// Host-side helpers of the path tracer: device allocation lists, the path workspace, scene
-// (shading data) upload and the camera basis.
+// Host-side helpers of the path tracer: device allocation lists, the path workspace, the
+// request-table ring, scene images and the camera basis.
 #include "pt_host.h"
 
 #include <algorithm>
@@ -44,14 +44,6 @@ int ws(DeviceAllocs& m, sortgpu_ctx* ctx, T** out, size_t count) {
     void* p = nullptr;
     int rc = m.alloc(ctx, &p, count * sizeof(T), "hipMalloc(path workspace)");
     *out = static_cast<T*>(p);
-    return rc;
-}
-
-template <class T>
-int up(DeviceAllocs& m, sortgpu_ctx* ctx, const T* src, size_t count, const T** dst, const char* what) {
-    void* p = nullptr;
-    int rc = m.upload(ctx, src, count * sizeof(T), &p, what);
-    *dst = static_cast<const T*>(p);
     return rc;
 }
 
@@ -147,8 +139,12 @@ int ReqRing::done(sortgpu_ctx* ctx, hipStream_t s) {
     return e == hipSuccess ? SORTGPU_OK : hip_fail(ctx, e, "hipEventRecord(request ring)");
 }
 
-int upload_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scene_desc* desc, SceneDev& h,
-                 DeviceAllocs& mem) {
+int prepare_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scene_desc* desc, bool with_tree,
+                  SceneImage& im) {
+    if (!acc || !desc) {
+        set_error(ctx, "scene: null argument");
+        return SORTGPU_ERR_ARG;
+    }
     const uint32_t ntris = (uint32_t)(acc->idx.size() / 3), nverts = (uint32_t)(acc->pos.size() / 3);
     const uint32_t nquads = (uint32_t)acc->quads.size();
     if (desc->nmaterials && !desc->materials) {
@@ -175,92 +171,152 @@ int upload_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scene
             set_error(ctx, "scene: material %u has unknown type %u", m, desc->materials[m].type);
             return SORTGPU_ERR_ARG;
         }
+    try {
+        im = SceneImage{};
+        im.with_tree = with_tree;
+        im.tree = acc->view;
+        im.tree.nodes = nullptr;
+        im.tree.prims = nullptr;
+        im.tree.quads = nullptr;
+        if (with_tree) {
+            im.nodes = acc->host.node_bytes;
+            im.prims = acc->host.prims;
+            im.quads = acc->host.quads;
+        }
+        im.idx = acc->idx;
+        im.pos = acc->pos;
+        if (desc->normals) im.nrm.assign(desc->normals, desc->normals + 3 * (size_t)nverts);
+        if (desc->tangents) im.tan.assign(desc->tangents, desc->tangents + 3 * (size_t)nverts);
+        if (desc->tri_material) im.tri_mat.assign(desc->tri_material, desc->tri_material + ntris);
 
-    // materials + the default white Lambert
-    std::vector<MatRec> mats(desc->nmaterials + 1);
-    std::memset(mats.data(), 0, mats.size() * sizeof(MatRec));
-    for (uint32_t m = 0; m < desc->nmaterials; m++) {
-        const sortgpu_material& s = desc->materials[m];
-        MatRec& d = mats[m];
-        d.type = s.type;
-        d.roughness = s.roughness;
-        d.ior = s.ior > 0.0f ? s.ior : 1.5f;
-        d.base = v3(s.base[0], s.base[1], s.base[2]);
-        d.trans = v3(s.trans[0], s.trans[1], s.trans[2]);
-        d.eta = v3(s.eta[0], s.eta[1], s.eta[2]);
-        d.k = v3(s.k[0], s.k[1], s.k[2]);
-        d.metallic = s.metallic;
-        d.specular = s.specular;
-        d.specular_tint = s.specular_tint;
-        d.sheen = s.sheen;
-        d.sheen_tint = s.sheen_tint;
-        d.clearcoat = s.clearcoat;
-        d.clearcoat_gloss = s.clearcoat_gloss;
-    }
-    mats.back().type = MAT_LAMBERT;
-    mats.back().base = v3(1.f, 1.f, 1.f);
+        // materials + the default white Lambert
+        im.mats.resize(desc->nmaterials + 1);
+        std::memset(im.mats.data(), 0, im.mats.size() * sizeof(MatRec));
+        for (uint32_t m = 0; m < desc->nmaterials; m++) {
+            const sortgpu_material& s = desc->materials[m];
+            MatRec& d = im.mats[m];
+            d.type = s.type;
+            d.roughness = s.roughness;
+            d.ior = s.ior > 0.0f ? s.ior : 1.5f;
+            d.base = v3(s.base[0], s.base[1], s.base[2]);
+            d.trans = v3(s.trans[0], s.trans[1], s.trans[2]);
+            d.eta = v3(s.eta[0], s.eta[1], s.eta[2]);
+            d.k = v3(s.k[0], s.k[1], s.k[2]);
+            d.metallic = s.metallic;
+            d.specular = s.specular;
+            d.specular_tint = s.specular_tint;
+            d.sheen = s.sheen;
+            d.sheen_tint = s.sheen_tint;
+            d.clearcoat = s.clearcoat;
+            d.clearcoat_gloss = s.clearcoat_gloss;
+        }
+        im.mats.back().type = MAT_LAMBERT;
+        im.mats.back().base = v3(1.f, 1.f, 1.f);
 
-    // lights: area quads in quad order, then the environment (Scene::genLightDistribution)
-    std::vector<LightRec> lights;
-    std::vector<float> power;
-    std::vector<int32_t> quad_light(nquads);
-    for (uint32_t qi = 0; qi < nquads; qi++) {
-        const sortgpu_quad& q = acc->quads[qi];
-        LightRec l;
-        std::memset(&l, 0, sizeof(l));
-        std::memcpy(l.mat, q.mat, 12 * sizeof(float));
-        std::memcpy(l.inv, q.inv, 16 * sizeof(float));
-        const float* Lq = desc->quad_radiance + 3 * qi;
-        l.L = v3(Lq[0], Lq[1], Lq[2]);
-        l.area = q.size_x * q.size_y;
-        l.half_x = q.size_x * 0.5f;
-        l.half_y = q.size_y * 0.5f;
-        l.prim = ntris + qi;
-        l.kind = 0;
-        quad_light[qi] = (int32_t)lights.size();
-        lights.push_back(l);
-        power.push_back(l.area * lum(Lq) * 6.28318530717958647692f);  // AreaLight::Power
-    }
-    int32_t env_idx = -1;
-    const float* E = desc->env_radiance;
-    if (E[0] != 0.0f || E[1] != 0.0f || E[2] != 0.0f) {
-        LightRec l;
-        std::memset(&l, 0, sizeof(l));
-        l.L = v3(E[0], E[1], E[2]);
-        l.kind = 1;
-        env_idx = (int32_t)lights.size();
-        lights.push_back(l);
-        const float* b = acc->host.bbox;
-        const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
-        const float r = std::sqrt(dx * dx + dy * dy + dz * dz) * 0.5f;
-        power.push_back(r * r * 3.14159265358979323846f * lum(E));  // SkyLight::Power
-    }
-    const uint32_t nl = (uint32_t)lights.size();
-    std::vector<float> cdf(nl + 1, 0.0f);  // Distribution1D (core/samplemethod.h:132-152)
-    for (uint32_t i = 0; i < nl; i++) cdf[i + 1] = cdf[i] + power[i];
-    const float sum = nl ? cdf[nl] : 0.0f;
-    for (uint32_t i = 0; i <= nl && nl; i++) cdf[i] = sum != 0.0f ? cdf[i] / sum : (float)i / (float)nl;
+        // lights: area quads in quad order, then the environment (Scene::genLightDistribution)
+        std::vector<float> power;
+        im.quad_light.resize(nquads);
+        for (uint32_t qi = 0; qi < nquads; qi++) {
+            const sortgpu_quad& q = acc->quads[qi];
+            LightRec l;
+            std::memset(&l, 0, sizeof(l));
+            std::memcpy(l.mat, q.mat, 12 * sizeof(float));
+            std::memcpy(l.inv, q.inv, 16 * sizeof(float));
+            const float* Lq = desc->quad_radiance + 3 * qi;
+            l.L = v3(Lq[0], Lq[1], Lq[2]);
+            l.area = q.size_x * q.size_y;
+            l.half_x = q.size_x * 0.5f;
+            l.half_y = q.size_y * 0.5f;
+            l.prim = ntris + qi;
+            l.kind = 0;
+            im.quad_light[qi] = (int32_t)im.lights.size();
+            im.lights.push_back(l);
+            power.push_back(l.area * lum(Lq) * 6.28318530717958647692f);  // AreaLight::Power
+        }
+        int32_t env_idx = -1;
+        const float* E = desc->env_radiance;
+        if (E[0] != 0.0f || E[1] != 0.0f || E[2] != 0.0f) {
+            LightRec l;
+            std::memset(&l, 0, sizeof(l));
+            l.L = v3(E[0], E[1], E[2]);
+            l.kind = 1;
+            env_idx = (int32_t)im.lights.size();
+            im.lights.push_back(l);
+            const float* b = acc->host.bbox;
+            const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+            const float r = std::sqrt(dx * dx + dy * dy + dz * dz) * 0.5f;
+            power.push_back(r * r * 3.14159265358979323846f * lum(E));  // SkyLight::Power
+        }
+        const uint32_t nl = (uint32_t)im.lights.size();
+        im.cdf.assign(nl + 1, 0.0f);  // Distribution1D (core/samplemethod.h:132-152)
+        for (uint32_t i = 0; i < nl; i++) im.cdf[i + 1] = im.cdf[i] + power[i];
+        const float sum = nl ? im.cdf[nl] : 0.0f;
+        for (uint32_t i = 0; i <= nl && nl; i++) im.cdf[i] = sum != 0.0f ? im.cdf[i] / sum : (float)i / (float)nl;
 
-    std::memset(&h, 0, sizeof(h));
-    h.tree = acc->view;
-    h.ntris = ntris;
-    h.nquads = nquads;
-    h.nlights = nl;
-    h.nmats = desc->nmaterials;
-    h.env_light = env_idx;
-    h.env_L = v3(E[0], E[1], E[2]);
-    int rc;
-    if ((rc = up(mem, ctx, acc->idx.data(), acc->idx.size(), &h.idx, "upload(idx)")) ||
-        (rc = up(mem, ctx, acc->pos.data(), acc->pos.size(), &h.pos, "upload(pos)")) ||
-        (rc = up(mem, ctx, desc->normals, desc->normals ? 3 * (size_t)nverts : 0, &h.nrm, "upload(normals)")) ||
-        (rc = up(mem, ctx, desc->tangents, desc->tangents ? 3 * (size_t)nverts : 0, &h.tan, "upload(tangents)")) ||
-        (rc = up(mem, ctx, desc->tri_material, desc->tri_material ? (size_t)ntris : 0, &h.tri_mat,
-                 "upload(materials)")) ||
-        (rc = up(mem, ctx, mats.data(), mats.size(), &h.mats, "upload(material table)")) ||
-        (rc = up(mem, ctx, lights.data(), lights.size(), &h.lights, "upload(lights)")) ||
-        (rc = up(mem, ctx, cdf.data(), cdf.size(), &h.light_cdf, "upload(light cdf)")) ||
-        (rc = up(mem, ctx, quad_light.data(), quad_light.size(), &h.quad_light, "upload(quad lights)")))
-        return rc;
+        SceneDev& h = im.dev;
+        std::memset(&h, 0, sizeof(h));
+        h.ntris = ntris;
+        h.nquads = nquads;
+        h.nlights = nl;
+        h.nmats = desc->nmaterials;
+        h.env_light = env_idx;
+        h.env_L = v3(E[0], E[1], E[2]);
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "scene: host allocation failed");
+        return SORTGPU_ERR_OOM;
+    }
+    return SORTGPU_OK;
+}
+
+namespace {
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <class T>
+size_t vbytes(const std::vector<T>& v) {
+    return v.size() * sizeof(T);
+}
+
+// Walks the arrays of an image in block order, calling f(host data, bytes, device pointer slot).
+template <class F>
+void for_each_array(const SceneImage& im, SceneDev& d, F f) {
+    if (im.with_tree) {
+        f(im.nodes.data(), vbytes(im.nodes), reinterpret_cast<const void**>(&d.tree.nodes));
+        f(im.prims.data(), vbytes(im.prims), reinterpret_cast<const void**>(&d.tree.prims));
+        f(im.quads.data(), vbytes(im.quads), reinterpret_cast<const void**>(&d.tree.quads));
+    }
+    f(im.idx.data(), vbytes(im.idx), reinterpret_cast<const void**>(&d.idx));
+    f(im.pos.data(), vbytes(im.pos), reinterpret_cast<const void**>(&d.pos));
+    f(im.nrm.data(), vbytes(im.nrm), reinterpret_cast<const void**>(&d.nrm));
+    f(im.tan.data(), vbytes(im.tan), reinterpret_cast<const void**>(&d.tan));
+    f(im.tri_mat.data(), vbytes(im.tri_mat), reinterpret_cast<const void**>(&d.tri_mat));
+    f(im.mats.data(), vbytes(im.mats), reinterpret_cast<const void**>(&d.mats));
+    f(im.lights.data(), vbytes(im.lights), reinterpret_cast<const void**>(&d.lights));
+    f(im.cdf.data(), vbytes(im.cdf), reinterpret_cast<const void**>(&d.light_cdf));
+    f(im.quad_light.data(), vbytes(im.quad_light), reinterpret_cast<const void**>(&d.quad_light));
+}
+}  // namespace
+
+size_t SceneImage::block_bytes() const {
+    size_t total = 0;
+    SceneDev d{};
+    for_each_array(*this, d, [&](const void*, size_t n, const void**) { total += align256(n); });
+    return std::max<size_t>(total, 256);
+}
+
+int SceneImage::place(sortgpu_ctx* ctx, void* block, SceneDev& out, hipStream_t s) const {
+    out = dev;
+    out.tree = tree;
+    size_t off = 0;
+    hipError_t e = hipSuccess;
+    char* base = static_cast<char*>(block);
+    for_each_array(*this, out, [&](const void* src, size_t n, const void** slot) {
+        *slot = nullptr;
+        if (!n) return;
+        *slot = base + off;
+        if (e == hipSuccess) e = hipMemcpyAsync(base + off, src, n, hipMemcpyHostToDevice, s);
+        off += align256(n);
+    });
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(scene)");
     return SORTGPU_OK;
 }
 

@@ -1,5 +1,6 @@
 // Host-side helpers shared by the single-scene render API and the serving engine: device
-// allocation lists, the path workspace, scene (shading data) upload and the camera basis.
+// allocation lists, the path workspace, the request-table ring, scene images (everything a
+// renderable scene puts in HBM, prepared on the host) and the camera basis.
 #pragma once
 #include <hip/hip_runtime_api.h>
 
@@ -48,10 +49,34 @@ struct ReqRing {
     int done(sortgpu_ctx* ctx, hipStream_t s);
 };
 
-// Validates `desc` against the accelerator's geometry and uploads the shading data; `out.tree`
-// is the accelerator's own tree view (its device records). Returns a status.
-int upload_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scene_desc* desc, SceneDev& out,
-                 DeviceAllocs& mem);
+// Everything one renderable scene puts in HBM, prepared on the host: optionally its tree records,
+// and the shading data (geometry by primitive id, materials + the default white Lambert, light
+// records in Scene::genLightDistribution order, the light-pick CDF, the quad -> light map).
+struct SceneImage {
+    bool with_tree = false;
+    TreeView tree{};                    // pointers unset
+    std::vector<unsigned char> nodes;
+    std::vector<PrimRec> prims;
+    std::vector<QuadRec> quads;
+    std::vector<uint32_t> idx, tri_mat;
+    std::vector<float> pos, nrm, tan, cdf;
+    std::vector<MatRec> mats;
+    std::vector<LightRec> lights;
+    std::vector<int32_t> quad_light;
+    SceneDev dev{};                     // counts and environment; pointers unset
+    // Bytes of the device block (every array 256-byte aligned).
+    size_t block_bytes() const;
+    // Copies the arrays into `block` (a device allocation of block_bytes()) and fills `out` with
+    // their device addresses; without a tree of its own, out.tree's record pointers stay null
+    // for the caller to set.
+    // The copies are enqueued on `s`; the image must stay alive until they complete.
+    int place(sortgpu_ctx* ctx, void* block, SceneDev& out, hipStream_t s) const;
+};
+
+// Validates `desc` against the accelerator's geometry and prepares the scene image (with a copy of
+// the accelerator's tree records when with_tree is set). Host-only: no device calls.
+int prepare_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scene_desc* desc, bool with_tree,
+                  SceneImage& out);
 
 // Camera basis of PerspectiveCamera (camera/perspective.cpp:26-64; math/transform.h).
 int make_camera(sortgpu_ctx* ctx, const sortgpu_camera* cam, uint32_t width, uint32_t height, CamDev& out);

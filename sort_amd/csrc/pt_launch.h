@@ -29,6 +29,11 @@ hipError_t pt_zero_f4(float4* a, uint32_t n, hipStream_t s);
 hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s);
 // raygen + bounce loop + finalize for the chunks of one wave.
 hipError_t pt_wave(const PtLaunch& L, const ChunkTable& ct, hipStream_t s);
+// The two halves of pt_wave: the paths (raygen + bounce loop, touching only the workspace) and
+// the film update (adding each chunk's samples, in sample order, to its request's film). Waves
+// on different streams may overlap their paths; their film updates must run in wave order.
+hipError_t pt_wave_paths(const PtLaunch& L, const ChunkTable& ct, hipStream_t s);
+hipError_t pt_wave_film(const PtLaunch& L, const ChunkTable& ct, hipStream_t s);
 hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStream_t s);
 hipError_t pt_li_rays(const PtLaunch& L, const float4* rays, const uint32_t* keys, uint32_t n, float* out,
                       hipStream_t s);

@@ -18,7 +18,7 @@ struct TraceArgs {
     unsigned long long* stats;     // 4 counters (device), only with stats
     // multi-scene form: trees != null selects it
     const TreeView* trees;         // device table of tree views
-    const uint32_t* ray_scene;     // per ray: index into trees (out-of-range ids miss, error bit 2)
+    const uint32_t* ray_scene;     // per ray: index into trees (out-of-range ids miss, error bit1)
     uint32_t n_trees;
     uint32_t width;                // node width shared by every tree of the table
     uint32_t max_stack;            // deepest worst-case stack over the table

@@ -75,6 +75,31 @@ def camera(eye, target, up=(0, 1, 0), fov_y_deg=40.0):
     return c
 
 
+def scene_desc(materials, tri_material=None, normals=None, tangents=None, quad_radiance=None,
+               env_radiance=(0.0, 0.0, 0.0)):
+    """A ``sortgpu_scene_desc`` and the arrays it points into (keep them alive while it is used)."""
+    keep = []
+    d = _lib.SceneDesc()
+
+    def arr(a, dt):
+        if a is None:
+            return None
+        x = np.ascontiguousarray(np.asarray(a, dt))
+        keep.append(x)
+        return x.ctypes.data
+
+    d.normals = arr(normals, np.float32)
+    d.tangents = arr(tangents, np.float32)
+    d.tri_material = arr(tri_material, np.uint32)
+    mats = material_array(materials)
+    keep.append(mats)
+    d.materials = ctypes.cast(mats, ctypes.c_void_p)
+    d.nmaterials = len(materials)
+    d.quad_radiance = arr(quad_radiance, np.float32)
+    d.env_radiance[:] = [float(x) for x in env_radiance]
+    return d, keep
+
+
 class PathTracing:
     """Shading data + path tracer over a built accelerator (``sort_amd.accel.Accelerator``)."""
 
@@ -82,24 +107,7 @@ class PathTracing:
                  env_radiance=(0.0, 0.0, 0.0), max_recursive_depth=6):
         self.accel = accel
         self.max_recursive_depth = max_recursive_depth
-        self._keep = []
-        d = _lib.SceneDesc()
-
-        def arr(a, dt, shape_last=None):
-            if a is None:
-                return None
-            x = np.ascontiguousarray(np.asarray(a, dt))
-            self._keep.append(x)
-            return x.ctypes.data
-
-        d.normals = arr(normals, np.float32)
-        d.tangents = arr(tangents, np.float32)
-        d.tri_material = arr(tri_material, np.uint32)
-        self._mats = material_array(materials)
-        d.materials = ctypes.cast(self._mats, ctypes.c_void_p)
-        d.nmaterials = len(materials)
-        d.quad_radiance = arr(quad_radiance, np.float32)
-        d.env_radiance[:] = [float(x) for x in env_radiance]
+        d, self._keep = scene_desc(materials, tri_material, normals, tangents, quad_radiance, env_radiance)
         h = ctypes.c_void_p()
         check(lib().sortgpu_scene_create(accel.handle, ctypes.byref(d), ctypes.byref(h)), accel.ctx.handle)
         self.handle = h
