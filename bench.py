@@ -16,6 +16,15 @@ the launch stream).
 
 cpu_baseline: the oracle (C++ AVX restatement of SORT's Obvh traversal, std::thread pool, 4096-ray
 chunks) on the same incoherent rays, on this host's cores.
+
+Further legs, reported beside the headline under their own keys (--legs selects them):
+  pt       -- the wavefront path tracer: the Cornell box (256^2, 16 spp, depth 5) and the 2M-triangle
+              spheres scene (1024^2, 64 spp, depth 8), samples/s and Mrays/s, with the oracle's CPU
+              path tracer timed on a bounded crop of each;
+  batched  -- 64 scenes in one concatenated pool, one persistent launch over all their incoherent
+              rays, against one launch per scene;
+  served   -- the render server under Poisson arrivals (20 requests/s over 16 resident scenes) and
+              saturated, p50/p99 latency and throughput against one request at a time.
 """
 from __future__ import annotations
 
@@ -42,7 +51,95 @@ def parse():
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tris", type=int, default=1 << 20)
+    p.add_argument("--legs", default="intersection,pt,batched,served")
+    p.add_argument("--served-requests", type=int, default=60)
     return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def leg_pt(ctx, cpu=True):
+    """Path tracer throughput on the Cornell box and on the 2M-triangle spheres scene."""
+    import torch
+
+    from sort_amd import Obvh, integrator as I, scenes
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    out = {}
+    cases = [("cornell", scenes.cornell_sphere(), 256, 256, 16, 5, 3),
+             ("spheres", scenes.spheres_scene(), 1024, 1024, 64, 8, 1)]
+    for name, sc, W, H, spp, depth, reps in cases:
+        t = time.perf_counter()
+        acc = Obvh(ctx).build(sc["pos"], sc["idx"], sc["quads"])
+        pt = I.PathTracing(acc, sc["materials"], tri_material=sc.get("tri_material"), normals=sc.get("normals"),
+                           quad_radiance=sc["quad_radiance"], env_radiance=sc["env"])
+        cam = I.camera(**sc["camera"])
+        build_s = time.perf_counter() - t
+        pt.render(cam, W, H, 1, seed=99, max_depth=depth)
+        torch.cuda.synchronize()
+        secs, ext, shadow = [], 0, 0
+        for r in range(reps):
+            t = time.perf_counter()
+            pt.render(cam, W, H, spp, seed=1 + r, max_depth=depth)
+            torch.cuda.synchronize()
+            secs.append(time.perf_counter() - t)
+            st = pt.stats()
+            ext += st["extension_rays"]
+            shadow += st["shadow_rays"]
+        sec = float(np.median(secs))
+        paths = W * H * spp
+        res = {"size": f"{W}x{H}", "spp": spp, "depth": depth, "tris": int(len(sc["idx"])),
+               "ms": round(sec * 1e3, 2), "msamples_s": round(paths / sec / 1e6, 2),
+               "mrays_s": round((ext + shadow) / reps / sec / 1e6, 2),
+               "rays_per_path": round((ext + shadow) / reps / paths, 3), "build_s": round(build_s, 2)}
+        if cpu:
+            from oracle.oracle import OraclePT
+            o = OraclePT(sc)
+            crop_rows = H if name == "cornell" else 32
+            ospp = spp if name == "cornell" else 4
+            st = {}
+            o.render(sc["camera"], W, H, ospp, depth, 1, threads=threads, crop=(0, W * crop_rows), stats=st)
+            cpaths = W * crop_rows * ospp
+            res["cpu_baseline"] = {"msamples_s": round(cpaths / st["seconds"] / 1e6, 3), "cores": threads,
+                                   "kind": "port", "sample": f"{W}x{crop_rows} pixels at {ospp} spp, depth {depth}"}
+            res["gpu_over_cpu"] = round(res["msamples_s"] / res["cpu_baseline"]["msamples_s"], 1)
+        log(f"[pt] {name}: {res}")
+        out[name] = res
+        pt.close()
+    return out
+
+
+def leg_batched(ctx):
+    """64 scenes, one pooled launch over all their rays versus one launch per scene."""
+    import torch
+
+    from sort_amd import autotune, batch
+    t = time.perf_counter()
+    w = batch.build_workload(ctx, n_scenes=64, width=8)
+    build_s = time.perf_counter() - t
+    pool = batch.Pool(ctx, w["accels"])
+    d_rays = torch.from_numpy(w["rays"]).cuda()
+    d_sid = torch.from_numpy(w["scene_of_ray"].view(np.int32)).cuda()
+    out = torch.empty((len(w["rays"]), 4), dtype=torch.float32, device="cuda")
+
+    def run_pool():
+        pool.trace(d_rays, d_sid, out=out)
+
+    def run_each():
+        for i, acc in enumerate(w["accels"]):
+            lo, hi = w["offsets"][i], w["offsets"][i + 1]
+            acc.get_intersect(d_rays[lo:hi])
+
+    n = len(w["rays"])
+    ms_pool = autotune._time(run_pool, reps=5, warmup=2)
+    ms_each = autotune._time(run_each, reps=5, warmup=2)
+    res = {"scenes": 64, "tris": int(w["tris"].sum()), "rays": n, "pool_mrays_s": round(n / ms_pool / 1e3, 1),
+           "per_scene_mrays_s": round(n / ms_each / 1e3, 1), "speedup": round(ms_each / ms_pool, 2),
+           "pool_bytes": pool.info()["device_bytes"], "build_s": round(build_s, 1)}
+    log(f"[batched] {res}")
+    pool.close()
+    return res
 
 
 def timed(fn, steps, warmup, stream):
@@ -67,7 +164,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
-                                   "(rays shard across GPUs with a replicated BVH, see DESIGN.md)"}))
+                                   "(rays shard across GPUs with a replicated BVH)"}))
         sys.exit(2)
     import torch
 
@@ -176,6 +273,17 @@ def main():
     }
     if cpu:
         out["config"]["gpu_over_cpu"] = round(inc_mrays / cpu["value"], 2)
+    legs = set(a.legs.split(","))
+    del d_inc, d_sh, d_cam, hits, occ
+    acc.release()
+    torch.cuda.empty_cache()
+    if "pt" in legs:
+        out["pt"] = leg_pt(ctx, cpu=not a.no_cpu)
+    if "batched" in legs:
+        out["batched"] = leg_batched(ctx)
+    if "served" in legs:
+        from sort_amd.serve import bench_served
+        out["served"] = bench_served(ctx, n_requests=a.served_requests, log=log)
     print(json.dumps(out))
 
 

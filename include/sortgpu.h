@@ -236,7 +236,8 @@ typedef struct sortgpu_server_opts {
                                 path key) for tests of the scheduling and failure handling */
     uint32_t fail_alloc_at;  /* test hook: the n-th budgeted allocation (scene load or film)
                                 fails as if out of memory (0 = never) */
-    uint32_t reserved[2];
+    uint32_t graphs;         /* 1: replay each wave's bounce loop from a captured HIP graph */
+    uint32_t reserved;
 } sortgpu_server_opts;
 
 typedef struct sortgpu_request {

@@ -1,7 +1,7 @@
 """ctypes binding of libsortgpu.so (the C ABI declared in include/sortgpu.h).
 
-The library is loaded from this package directory only. A missing library is an error: there
-is no fallback implementation.
+The library is loaded from this package directory (or from the path in SORTGPU_LIB). A missing
+library is an error: there is no fallback implementation.
 """
 from __future__ import annotations
 
@@ -10,7 +10,8 @@ import os
 import re
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libsortgpu.so")
+# SORTGPU_LIB names another build of the same library (e.g. a kernel variant being measured).
+LIB_PATH = os.environ.get("SORTGPU_LIB") or os.path.join(PKG, "libsortgpu.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "sortgpu.h")
 
 OK = 0
@@ -64,7 +65,7 @@ class Camera(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("wave_paths", ctypes.c_uint32), ("max_active", ctypes.c_uint32),
                 ("inflight_waves", ctypes.c_uint32), ("manual", ctypes.c_uint32), ("host_device", ctypes.c_uint32),
-                ("fail_alloc_at", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("fail_alloc_at", ctypes.c_uint32), ("graphs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):

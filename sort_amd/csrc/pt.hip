@@ -73,13 +73,13 @@ __device__ __forceinline__ int chunk_of_path(const ChunkTable& ct, uint32_t p) {
 // Camera rays of every chunk (PerspectiveCamera::GenerateRay in closed form: raster -> camera
 // through the inverse of Scale(W,H,1) Scale(.5,-.5,1) Translate(1,-1,0) Perspective(xs, ys),
 // then the orthonormal ViewLookat basis).
-__global__ __launch_bounds__(kPtBlock) void k_raygen(const ChunkTable ct, const ReqDev* __restrict__ reqs,
-                                                     PathState ps, Queues q) {
+__global__ __launch_bounds__(kPtBlock) void k_raygen(const WaveDesc* __restrict__ w, PathState ps, Queues q) {
+    const ChunkTable& ct = w->ct;
     const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
     if (p == 0) q.count[0] = ct.total_paths;
     if (p >= ct.total_paths) return;
     const ChunkDev& c = ct.c[chunk_of_path(ct, p)];
-    const ReqDev& R = reqs[c.req];
+    const ReqDev& R = w->reqs[c.req];
     const uint32_t local = p - c.path_base;
     const uint32_t pix = local % c.npix;
     const uint32_t s = c.sample0 + local / c.npix;
@@ -376,8 +376,9 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve(PathState ps, Queues q) {
 // ---- finalize ----------------------------------------------------------------------------------
 // One thread per (chunk, pixel): adds the chunk's samples of that pixel, in sample order, to the
 // request's film (a request has at most one chunk per wave, so no two threads share a pixel).
-__global__ __launch_bounds__(kPtBlock) void k_finalize(const ChunkTable ct, const ReqDev* __restrict__ reqs,
-                                                       PathState ps) {
+__global__ __launch_bounds__(kPtBlock) void k_finalize(const WaveDesc* __restrict__ w, PathState ps) {
+    const ChunkTable& ct = w->ct;
+    const ReqDev* reqs = w->reqs;
     const uint32_t g = blockIdx.x * kPtBlock + threadIdx.x;
     if (g >= ct.total_pix) return;
     int k = 0;
@@ -457,7 +458,8 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
         }
         if ((e = launch_trace(ta, false, false, s, L.num_cus)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, L.q.count, nxt);
-        hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.reqs, b, cur, L.ps, L.q);
+        hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.wave->reqs, b, cur, L.ps,
+                           L.q);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         TraceArgs sa = ta;
         sa.rays = L.q.sray;
@@ -473,23 +475,23 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t pt_wave_paths(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
-    if (ct.total_paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(ct.total_paths)), dim3(kPtBlock), 0, s, ct, L.reqs, L.ps, L.q);
+hipError_t pt_wave_paths(const PtLaunch& L, hipStream_t s) {
+    if (L.paths_cap == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_raygen, dim3(blocks_for(L.paths_cap)), dim3(kPtBlock), 0, s, L.wave, L.ps, L.q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return pt_bounces(L, ct.total_paths, s);
+    return pt_bounces(L, L.paths_cap, s);
 }
 
-hipError_t pt_wave_film(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
-    if (ct.total_pix == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(ct.total_pix)), dim3(kPtBlock), 0, s, ct, L.reqs, L.ps);
+hipError_t pt_wave_film(const PtLaunch& L, hipStream_t s) {
+    if (L.pix_cap == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(L.pix_cap)), dim3(kPtBlock), 0, s, L.wave, L.ps);
     return hipGetLastError();
 }
 
-hipError_t pt_wave(const PtLaunch& L, const ChunkTable& ct, hipStream_t s) {
-    hipError_t e = pt_wave_paths(L, ct, s);
-    return e != hipSuccess ? e : pt_wave_film(L, ct, s);
+hipError_t pt_wave(const PtLaunch& L, hipStream_t s) {
+    hipError_t e = pt_wave_paths(L, s);
+    return e != hipSuccess ? e : pt_wave_film(L, s);
 }
 
 hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStream_t s) {
@@ -498,7 +500,8 @@ hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStr
 }
 
 hipError_t pt_li_rays(const PtLaunch& L, const float4* rays, const uint32_t* keys, uint32_t n, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_raygen_rays, dim3(blocks_for(n)), dim3(kPtBlock), 0, s, rays, keys, n, L.reqs, L.ps, L.q);
+    hipLaunchKernelGGL(k_raygen_rays, dim3(blocks_for(n)), dim3(kPtBlock), 0, s, rays, keys, n, L.wave->reqs, L.ps,
+                       L.q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if ((e = pt_bounces(L, n, s)) != hipSuccess) return e;

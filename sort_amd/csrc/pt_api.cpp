@@ -40,7 +40,7 @@ int ensure_film(sortgpu_scene* sc, uint32_t npix) {
     return SORTGPU_OK;
 }
 
-PtLaunch make_launch(sortgpu_scene* sc, const ReqDev* d_reqs, uint32_t max_depth) {
+PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_depth) {
     PtLaunch L;
     std::memset(&L, 0, sizeof(L));
     sortgpu_accel* acc = sc->accel;
@@ -51,7 +51,7 @@ PtLaunch make_launch(sortgpu_scene* sc, const ReqDev* d_reqs, uint32_t max_depth
     L.trace.stats = nullptr;
     apply_tuning(acc->ctx, L.trace);
     L.scenes = sc->d_scene;
-    L.reqs = d_reqs;
+    L.wave = d_wave;
     L.trees = nullptr;
     L.ps = sc->ws.ps;
     L.q = sc->ws.q;
@@ -146,8 +146,9 @@ int sortgpu_render(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width,
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t npix = width * height;
     const uint32_t spw = samples_per_wave(npix, spp);
-    ReqDev R;
-    std::memset(&R, 0, sizeof(R));
+    WaveDesc wd;
+    std::memset(&wd, 0, sizeof(wd));
+    ReqDev& R = wd.reqs[0];
     if ((rc = make_camera(ctx, cam, width, height, R.cam))) return rc;
     if ((rc = sc->ws.ensure(ctx, npix * spw)) || (rc = ensure_film(sc, npix))) return rc;
     R.scene = 0;
@@ -155,26 +156,26 @@ int sortgpu_render(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width,
     R.max_depth = max_depth;
     R.npix = npix;
     R.film = sc->film;
-    const ReqDev* d_reqs = nullptr;
     hipError_t e = hipEventRecord(sc->ev0, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
-    if ((rc = sc->ring.push(ctx, &R, 1, s, &d_reqs))) return rc;
-    PtLaunch L = make_launch(sc, d_reqs, max_depth);
     if ((e = pt_zero_f4(sc->film, npix, s)) != hipSuccess) return hip_fail(ctx, e, "zero film");
     if ((e = pt_zero_f4(reinterpret_cast<float4*>(sc->ws.q.totals), 1, s)) != hipSuccess)
         return hip_fail(ctx, e, "zero totals");
     for (uint32_t s0 = 0; s0 < spp; s0 += spw) {
         const uint32_t ns = std::min(spw, spp - s0);
-        ChunkTable ct;
-        std::memset(&ct, 0, sizeof(ct));
-        ct.n = 1;
-        ct.total_paths = npix * ns;
-        ct.total_pix = npix;
-        ct.c[0] = ChunkDev{0u, s0, ns, npix, 0u, 0u};
-        if ((e = pt_wave(L, ct, s)) != hipSuccess) return hip_fail(ctx, e, "render wave");
+        wd.ct.n = 1;
+        wd.ct.total_paths = npix * ns;
+        wd.ct.total_pix = npix;
+        wd.ct.c[0] = ChunkDev{0u, s0, ns, npix, 0u, 0u};
+        const WaveDesc* d_wave = nullptr;
+        if ((rc = sc->ring.push(ctx, wd, s, nullptr, &d_wave))) return rc;
+        PtLaunch L = make_launch(sc, d_wave, max_depth);
+        L.paths_cap = npix * ns;
+        L.pix_cap = npix;
+        if ((e = pt_wave(L, s)) != hipSuccess) return hip_fail(ctx, e, "render wave");
+        if ((rc = sc->ring.done(ctx, s))) return rc;
     }
     if ((e = pt_resolve_film(sc->film, npix, d_film_rgb, s)) != hipSuccess) return hip_fail(ctx, e, "resolve film");
-    if ((rc = sc->ring.done(ctx, s))) return rc;
     if ((e = hipEventRecord(sc->ev1, s)) != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
     sc->last_paths = (uint64_t)npix * spp;
     return SORTGPU_OK;
@@ -196,16 +197,18 @@ int sortgpu_li(sortgpu_scene* sc, const sortgpu_ray* d_rays, const uint32_t* d_k
     if (rc) return rc;
     if ((rc = sc->ws.ensure(ctx, n))) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    ReqDev R;
-    std::memset(&R, 0, sizeof(R));
+    WaveDesc wd;
+    std::memset(&wd, 0, sizeof(wd));
+    wd.ct.n = 1;
+    ReqDev& R = wd.reqs[0];
     R.scene = 0;
     R.seed = fold_seed(seed);
     R.max_depth = max_depth;
-    const ReqDev* d_reqs = nullptr;
+    const WaveDesc* d_wave = nullptr;
     hipError_t e = hipEventRecord(sc->ev0, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
-    if ((rc = sc->ring.push(ctx, &R, 1, s, &d_reqs))) return rc;
-    PtLaunch L = make_launch(sc, d_reqs, max_depth);
+    if ((rc = sc->ring.push(ctx, wd, s, nullptr, &d_wave))) return rc;
+    PtLaunch L = make_launch(sc, d_wave, max_depth);
     if ((e = pt_zero_f4(reinterpret_cast<float4*>(sc->ws.q.totals), 1, s)) != hipSuccess)
         return hip_fail(ctx, e, "zero totals");
     if ((e = pt_li_rays(L, reinterpret_cast<const float4*>(d_rays), d_keys, n, d_rgb, s)) != hipSuccess)

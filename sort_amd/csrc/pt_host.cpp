@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
 namespace sortgpu {
@@ -82,7 +83,7 @@ void PathWorkspace::release() {
 }
 
 int ReqRing::init(sortgpu_ctx* ctx) {
-    const size_t n = (size_t)kSlots * kMaxChunks * sizeof(ReqDev);
+    const size_t n = (size_t)kSlots * sizeof(WaveDesc);
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host), n, hipHostMallocDefault);
     if (e != hipSuccess) {
         host = nullptr;
@@ -115,22 +116,24 @@ void ReqRing::release() {
     dev = nullptr;
 }
 
-int ReqRing::push(sortgpu_ctx* ctx, const ReqDev* r, uint32_t n, hipStream_t s, const ReqDev** d_table) {
+int ReqRing::push(sortgpu_ctx* ctx, const WaveDesc& w, hipStream_t s, WaveDesc* dst, const WaveDesc** d_desc) {
+    const uint32_t n = w.ct.n;
     if (n == 0 || n > (uint32_t)kMaxChunks) {
-        set_error(ctx, "request table: %u requests (1..%d)", n, kMaxChunks);
+        set_error(ctx, "wave descriptor: %u requests (1..%d)", n, kMaxChunks);
         return SORTGPU_ERR_ARG;
     }
     const uint32_t slot = next;
     next = (next + 1) % kSlots;
     hipError_t e = hipEventSynchronize(ev[slot]);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventSynchronize(request ring)");
-    ReqDev* h = host + (size_t)slot * kMaxChunks;
-    ReqDev* d = dev + (size_t)slot * kMaxChunks;
-    std::memcpy(h, r, n * sizeof(ReqDev));
-    if ((e = hipMemcpyAsync(d, h, n * sizeof(ReqDev), hipMemcpyHostToDevice, s)) != hipSuccess)
-        return hip_fail(ctx, e, "hipMemcpyAsync(request table)");
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventSynchronize(wave ring)");
+    WaveDesc* h = host + slot;
+    WaveDesc* d = dst ? dst : dev + slot;
+    const size_t bytes = offsetof(WaveDesc, reqs) + n * sizeof(ReqDev);
+    std::memcpy(h, &w, bytes);
+    if ((e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(ctx, e, "hipMemcpyAsync(wave descriptor)");
     last = slot;
-    *d_table = d;
+    *d_desc = d;
     return SORTGPU_OK;
 }
 

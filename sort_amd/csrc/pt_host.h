@@ -33,19 +33,20 @@ struct PathWorkspace {
     static size_t bytes_for(uint32_t paths);
 };
 
-// Request tables of waves in flight: a ring of slots, each a pinned host staging table and its
-// device copy. push() waits until the slot's previous wave has finished with it, stages `n`
-// records, copies them to the device on `s` and returns the device table; done() records that
-// the wave that reads the slot has been enqueued on `s`.
+// Wave descriptors of waves in flight: a ring of slots, each a pinned host staging descriptor and
+// a device copy. push() waits until the slot's previous wave has finished with it, stages `w`
+// (its chunk table and the first w.ct.n requests), copies it on `s` to `dst` (or, when null, to
+// the slot's own device copy) and returns the device address; done() records that the wave that
+// reads the slot has been enqueued on `s`.
 struct ReqRing {
     static constexpr int kSlots = 8;
-    ReqDev* host = nullptr;
-    ReqDev* dev = nullptr;
+    WaveDesc* host = nullptr;
+    WaveDesc* dev = nullptr;
     hipEvent_t ev[kSlots] = {};
     uint32_t next = 0, last = 0;
     int init(sortgpu_ctx* ctx);
     void release();
-    int push(sortgpu_ctx* ctx, const ReqDev* r, uint32_t n, hipStream_t s, const ReqDev** d_table);
+    int push(sortgpu_ctx* ctx, const WaveDesc& w, hipStream_t s, WaveDesc* dst, const WaveDesc** d_desc);
     int done(sortgpu_ctx* ctx, hipStream_t s);
 };
 

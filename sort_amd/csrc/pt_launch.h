@@ -11,7 +11,7 @@ namespace sortgpu {
 struct PtLaunch {
     TraceArgs trace;            // counter / error slots and tuning; the single tree when trees == null
     const SceneDev* scenes;     // device scene table
-    const ReqDev* reqs;         // device request table
+    const WaveDesc* wave;       // device wave descriptor (chunks + request table)
     const TreeView* trees;      // device tree table for multi-scene waves, or null
     uint32_t n_trees;
     uint32_t tree_width;        // node width of the tree table
@@ -19,6 +19,8 @@ struct PtLaunch {
     PathState ps;
     Queues q;
     uint32_t max_depth;         // deepest request of the wave
+    uint32_t paths_cap;         // grid bound: at least the wave's paths (kernels read the exact count)
+    uint32_t pix_cap;           // grid bound: at least the wave's film pixels
     int num_cus;
 };
 
@@ -28,13 +30,16 @@ constexpr size_t kPathBytes = 4 * 16 + 3 * 4 + 2 * (32 + 4 + 4) + 16 + 2 * 32 + 
 hipError_t pt_zero_f4(float4* a, uint32_t n, hipStream_t s);
 hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s);
 // raygen + bounce loop + finalize for the chunks of one wave.
-hipError_t pt_wave(const PtLaunch& L, const ChunkTable& ct, hipStream_t s);
+hipError_t pt_wave(const PtLaunch& L, hipStream_t s);
 // The two halves of pt_wave: the paths (raygen + bounce loop, touching only the workspace) and
 // the film update (adding each chunk's samples, in sample order, to its request's film). Waves
-// on different streams may overlap their paths; their film updates must run in wave order.
-hipError_t pt_wave_paths(const PtLaunch& L, const ChunkTable& ct, hipStream_t s);
-hipError_t pt_wave_film(const PtLaunch& L, const ChunkTable& ct, hipStream_t s);
+// on different streams may overlap their paths; their film updates must run in wave order. The
+// paths half launches the same kernels with the same arguments for every wave of a given
+// workspace, descriptor address, depth and capacity, so it can be captured once as a graph.
+hipError_t pt_wave_paths(const PtLaunch& L, hipStream_t s);
+hipError_t pt_wave_film(const PtLaunch& L, hipStream_t s);
 hipError_t pt_resolve_film(const float4* film, uint32_t npix, float* rgb, hipStream_t s);
+// Caller rays of request 0 of L.wave's table.
 hipError_t pt_li_rays(const PtLaunch& L, const float4* rays, const uint32_t* keys, uint32_t n, float* out,
                       hipStream_t s);
 

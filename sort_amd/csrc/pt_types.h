@@ -53,6 +53,14 @@ struct ChunkTable {
     ChunkDev c[kMaxChunks];
 };
 
+// Everything a wave's kernels read about its work, in device memory: the chunks and the request
+// table they index. A wave's launches take only its address, so a captured wave graph replays for
+// any wave whose descriptor has been copied to the same address.
+struct WaveDesc {
+    ChunkTable ct;
+    ReqDev reqs[kMaxChunks];
+};
+
 // Path slots of one wave (fixed for the wave; index = path id).
 struct PathState {
     float4* T;        // throughput

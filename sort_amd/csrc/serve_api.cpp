@@ -54,7 +54,7 @@ int sortgpu_server_create(sortgpu_ctx* ctx, const sortgpu_server_opts* opts, sor
             set_error(ctx, "sortgpu_server_create: host-only context (use host_device for the stand-in)");
             return SORTGPU_ERR_STATE;
         }
-        int rc = serve::make_hip_device(ctx, wave_paths, kMaxServerScenes, eo.inflight, dev, err);
+        int rc = serve::make_hip_device(ctx, wave_paths, kMaxServerScenes, eo.inflight, o.graphs != 0, dev, err);
         if (rc) return fail(ctx, rc, "sortgpu_server_create", err);
     }
     sortgpu_server* s = new (std::nothrow) sortgpu_server();

@@ -43,8 +43,9 @@ class HipDevice : public Device {
   public:
     static constexpr int kMaxStreams = 4;
 
-    HipDevice(sortgpu_ctx* ctx, uint32_t wave_paths, uint32_t max_scenes, uint32_t nstreams)
-        : ctx_(ctx), cap_(wave_paths), max_scenes_(max_scenes), ns_(std::max(1u, std::min(nstreams, 4u))) {}
+    HipDevice(sortgpu_ctx* ctx, uint32_t wave_paths, uint32_t max_scenes, uint32_t nstreams, bool graphs)
+        : ctx_(ctx), cap_(wave_paths), max_scenes_(max_scenes), ns_(std::max(1u, std::min(nstreams, 4u))),
+          graphs_(graphs) {}
 
     int init(std::string& err) {
         int rc = ctx_bind(ctx_);
@@ -54,6 +55,10 @@ class HipDevice : public Device {
             if ((e = hipStreamCreateWithFlags(&s_[i], hipStreamNonBlocking)) != hipSuccess)
                 return (err = hip_msg("hipStreamCreate", e), hip_rc(e));
             if ((rc = ws_[i].ensure(ctx_, cap_))) return (err = "path workspace allocation failed", rc);
+            void* p = nullptr;
+            if ((e = hipMalloc(&p, sizeof(WaveDesc))) != hipSuccess)
+                return (err = hip_msg("hipMalloc(wave descriptor)", e), hip_rc(e));
+            d_desc_[i] = static_cast<WaveDesc*>(p);
         }
         if ((e = hipStreamCreateWithFlags(&s_up_, hipStreamNonBlocking)) != hipSuccess)
             return (err = hip_msg("hipStreamCreate", e), hip_rc(e));
@@ -87,14 +92,17 @@ class HipDevice : public Device {
             if ((e = pt_zero_f4(reinterpret_cast<float4*>(ws_[i].q.totals), 1, s_[i])) != hipSuccess ||
                 (e = hipStreamSynchronize(s_[i])) != hipSuccess)
                 return (err = hip_msg("workspace initialisation", e), hip_rc(e));
-        fixed_ = sb + tb + cb + 256 + (size_t)ReqRing::kSlots * kMaxChunks * sizeof(ReqDev);
-        for (uint32_t i = 0; i < ns_; i++) fixed_ += ws_[i].mem.bytes;
+        fixed_ = sb + tb + cb + 256 + (size_t)ReqRing::kSlots * sizeof(WaveDesc);
+        for (uint32_t i = 0; i < ns_; i++) fixed_ += ws_[i].mem.bytes + sizeof(WaveDesc);
         return SORTGPU_OK;
     }
 
     ~HipDevice() override {
         ctx_bind(ctx_);
         sync_all();
+        drop_graphs();
+        for (uint32_t i = 0; i < ns_; i++)
+            if (d_desc_[i]) hipFree(d_desc_[i]);
         for (SceneHost& sh : scenes_)
             if (sh.block) hipFree(sh.block);
         for (auto& kv : cache_) hipFree(kv.second);
@@ -215,10 +223,10 @@ class HipDevice : public Device {
 
     int launch(const Wave& w, std::string& err) override {
         ctx_bind(ctx_);
-        ReqDev table[kMaxChunks];
-        ChunkTable ct;
-        std::memset(table, 0, sizeof(table));
-        std::memset(&ct, 0, sizeof(ct));
+        WaveDesc wd;
+        std::memset(&wd, 0, sizeof(wd));
+        ChunkTable& ct = wd.ct;
+        ReqDev* table = wd.reqs;
         if (w.chunks.size() > (size_t)kMaxChunks) return (err = "too many requests in a wave", SORTGPU_ERR_ARG);
         uint32_t path_base = 0, pix_base = 0;
         for (size_t k = 0; k < w.chunks.size(); k++) {
@@ -248,31 +256,26 @@ class HipDevice : public Device {
         if (e == hipSuccess) e = hipStreamWaitEvent(s, ev_up_, 0);
         for (size_t c = 0; c < w.chunks.size() && e == hipSuccess; c++)
             if (w.chunks[c].sample0 == 0) e = pt_zero_f4(table[c].film, table[c].npix, s);
-        const ReqDev* d_reqs = nullptr;
+        // the stream's own descriptor: its previous wave (same stream) has finished reading it
+        const WaveDesc* d_wave = nullptr;
         int rc = SORTGPU_OK;
-        if (e == hipSuccess) rc = ring_.push(ctx_, table, ct.n, s, &d_reqs);
+        if (e == hipSuccess) rc = ring_.push(ctx_, wd, s, d_desc_[k], &d_wave);
         if (e == hipSuccess && rc == SORTGPU_OK) {
-            PtLaunch L;
-            std::memset(&L, 0, sizeof(L));
-            L.trace.counter = d_counters_ + (slot_++ % kCounterSlots) * kCounterStride;
-            L.trace.error = d_error_;
-            apply_tuning(ctx_, L.trace);
-            L.scenes = d_scenes_;
-            L.reqs = d_reqs;
-            L.trees = d_trees_;
-            L.n_trees = max_scenes_;
-            L.tree_width = width_ ? width_ : 8;
-            L.tree_max_stack = max_stack_;
-            L.ps = ws_[k].ps;
-            L.q = ws_[k].q;
-            L.max_depth = w.max_depth;
-            L.num_cus = ctx_->num_cus;
-            e = pt_wave_paths(L, ct, s);
+            PtLaunch L = wave_launch(k, w.max_depth);
+            L.paths_cap = ct.total_paths;
+            L.pix_cap = ct.total_pix;
+            if (graphs_) {
+                hipGraphExec_t x = nullptr;
+                if ((rc = wave_graph(k, w.max_depth, &x)) == SORTGPU_OK) e = hipGraphLaunch(x, s);
+            } else {
+                e = pt_wave_paths(L, s);
+            }
             // film updates in wave order: wait for the previous wave's
-            if (e == hipSuccess && film_recorded_) e = hipStreamWaitEvent(s, ev_film_, 0);
-            if (e == hipSuccess) e = pt_wave_film(L, ct, s);
-            if (e == hipSuccess && (e = hipEventRecord(ev_film_, s)) == hipSuccess) film_recorded_ = true;
-            if (e == hipSuccess) rc = ring_.done(ctx_, s);
+            if (e == hipSuccess && rc == SORTGPU_OK && film_recorded_) e = hipStreamWaitEvent(s, ev_film_, 0);
+            if (e == hipSuccess && rc == SORTGPU_OK) e = pt_wave_film(L, s);
+            if (e == hipSuccess && rc == SORTGPU_OK && (e = hipEventRecord(ev_film_, s)) == hipSuccess)
+                film_recorded_ = true;
+            if (e == hipSuccess && rc == SORTGPU_OK) rc = ring_.done(ctx_, s);
         }
         // the end event is recorded whatever happened, so the wave retires in order
         hipError_t e2 = hipEventRecord(ev.b, s);
@@ -335,6 +338,77 @@ class HipDevice : public Device {
         uint64_t seq;
         hipEvent_t a, b;
     };
+
+    // Launch parameters of the waves of stream k: its workspace, its descriptor and its own work
+    // counter (waves of one stream run one after another).
+    PtLaunch wave_launch(uint32_t k, uint32_t depth) const {
+        PtLaunch L;
+        std::memset(&L, 0, sizeof(L));
+        L.trace.counter = d_counters_ + k * kCounterStride;
+        L.trace.error = d_error_;
+        apply_tuning(ctx_, L.trace);
+        L.scenes = d_scenes_;
+        L.wave = d_desc_[k];
+        L.trees = d_trees_;
+        L.n_trees = max_scenes_;
+        L.tree_width = width_ ? width_ : 8;
+        L.tree_max_stack = max_stack_;
+        L.ps = ws_[k].ps;
+        L.q = ws_[k].q;
+        L.max_depth = depth;
+        L.num_cus = ctx_->num_cus;
+        return L;
+    }
+
+    // The paths half of a wave on stream k captured as a graph (raygen + `depth` bounces, grids
+    // sized for the workspace's capacity; the kernels read the wave's exact counts from its
+    // descriptor). Captured on first use per depth, again when a scene changes the tree table's
+    // width or stack depth or the context's tuning changes. The graph holds kernel launches only.
+    int wave_graph(uint32_t k, uint32_t depth, hipGraphExec_t* out) {
+        const sortgpu_tuning& t = ctx_->tuning;
+        const uint64_t stamp = ((uint64_t)width_ << 56) ^ ((uint64_t)max_stack_ << 40) ^ ((uint64_t)t.refill << 24) ^
+                               ((uint64_t)t.lds_entries << 12) ^ (uint64_t)t.blocks_per_cu;
+        if (stamp != graph_stamp_) {
+            drop_graphs();
+            graph_stamp_ = stamp;
+        }
+        auto it = gexec_[k].find(depth);
+        if (it != gexec_[k].end()) {
+            *out = it->second;
+            return SORTGPU_OK;
+        }
+        hipStream_t s = s_[k];
+        PtLaunch L = wave_launch(k, depth);
+        L.paths_cap = cap_;
+        L.pix_cap = cap_;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) return hip_fail(ctx_, e, "hipStreamBeginCapture(wave)");
+        const hipError_t e1 = pt_wave_paths(L, s);
+        e = hipStreamEndCapture(s, &g);
+        if (e1 != hipSuccess || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return hip_fail(ctx_, e1 != hipSuccess ? e1 : e, "wave graph capture");
+        }
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(ctx_, e, "hipGraphInstantiate(wave)");
+        gexec_[k][depth] = x;
+        *out = x;
+        return SORTGPU_OK;
+    }
+
+    void drop_graphs() {
+        bool any = false;
+        for (uint32_t i = 0; i < ns_; i++) any |= !gexec_[i].empty();
+        if (!any) return;
+        sync_all();
+        for (uint32_t i = 0; i < ns_; i++) {
+            for (auto& kv : gexec_[i]) hipGraphExecDestroy(kv.second);
+            gexec_[i].clear();
+        }
+    }
     hipEvent_t take_event() {
         if (!spare_.empty()) {
             hipEvent_t e = spare_.back();
@@ -352,6 +426,10 @@ class HipDevice : public Device {
 
     sortgpu_ctx* ctx_;
     uint32_t cap_, max_scenes_, ns_;
+    bool graphs_;
+    WaveDesc* d_desc_[kMaxStreams] = {};
+    std::map<uint32_t, hipGraphExec_t> gexec_[kMaxStreams];
+    uint64_t graph_stamp_ = 0;
     hipStream_t s_[kMaxStreams] = {};
     hipStream_t s_up_ = nullptr;
     hipEvent_t ev_up_ = nullptr, ev_film_ = nullptr;
@@ -465,9 +543,9 @@ class HostDevice : public Device {
 
 }  // namespace
 
-int make_hip_device(sortgpu_ctx* ctx, uint32_t wave_paths, uint32_t max_scenes, uint32_t streams,
+int make_hip_device(sortgpu_ctx* ctx, uint32_t wave_paths, uint32_t max_scenes, uint32_t streams, bool graphs,
                     std::unique_ptr<Device>& out, std::string& err) {
-    std::unique_ptr<HipDevice> d(new HipDevice(ctx, wave_paths, max_scenes, streams));
+    std::unique_ptr<HipDevice> d(new HipDevice(ctx, wave_paths, max_scenes, streams, graphs));
     int rc = d->init(err);
     if (rc) return rc;
     out = std::move(d);

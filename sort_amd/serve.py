@@ -39,10 +39,10 @@ class Server:
     """Render server on a device context (or, with host_device, a host-only stand-in device)."""
 
     def __init__(self, ctx, hbm_budget=0, wave_paths=0, max_active=0, inflight_waves=0, manual=False,
-                 host_device=False, fail_alloc_at=0):
+                 host_device=False, fail_alloc_at=0, graphs=False):
         self.ctx = ctx
         o = _lib.ServerOpts(int(hbm_budget), int(wave_paths), int(max_active), int(inflight_waves), int(manual),
-                            int(host_device), int(fail_alloc_at))
+                            int(host_device), int(fail_alloc_at), int(graphs))
         h = ctypes.c_void_p()
         check(lib().sortgpu_server_create(ctx.handle, ctypes.byref(o), ctypes.byref(h)), ctx.handle)
         self.handle = h

@@ -78,11 +78,15 @@ def test_served_request_matches_plain_render_and_oracle(gpu_ctx, world):
     srv.close()
 
 
-@pytest.mark.parametrize("wave_paths", [1 << 14, 1 << 18])
-def test_alone_equals_in_mix(gpu_ctx, world, wave_paths):
-    reqs = _mix(world, 16, seed=wave_paths)
-    srv, ids = _server(gpu_ctx, world, wave_paths=wave_paths, max_active=8)
-    print(f"[serve] mix of {len(reqs)} requests, wave {wave_paths}", flush=True)
+@pytest.mark.parametrize("wave_paths,graphs,streams", [(1 << 14, False, 2), (1 << 18, False, 2), (1 << 14, True, 2),
+                                                       (1 << 16, True, 3), (1 << 16, False, 1)])
+def test_alone_equals_in_mix(gpu_ctx, world, wave_paths, graphs, streams):
+    """Graph mode replays each captured bounce loop for many waves, with the server's own eager
+    work (descriptor copies, film clears and updates, result copies) between the replays; every
+    result is checked against the eager single-request render."""
+    reqs = _mix(world, 16, seed=wave_paths + streams)
+    srv, ids = _server(gpu_ctx, world, wave_paths=wave_paths, max_active=8, graphs=graphs, inflight_waves=streams)
+    print(f"[serve] mix of {len(reqs)} requests, wave {wave_paths}, graphs {graphs}, streams {streams}", flush=True)
     rids = [srv.submit(ids[r["k"]], r["cam"], r["w"], r["h"], r["spp"], r["depth"], r["seed"]) for r in reqs]
     for rid in rids:
         info = srv.wait(rid, 120.0)
