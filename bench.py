@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--tris", type=int, default=1 << 20)
     p.add_argument("--legs", default="intersection,pt,batched,served")
     p.add_argument("--served-requests", type=int, default=60)
+    p.add_argument("--copy-probe", action="store_true",
+                   help="end with three 256 MiB device copies (HBM counter calibration)")
     return p.parse_args()
 
 
@@ -284,6 +286,15 @@ def main():
     if "served" in legs:
         from sort_amd.serve import bench_served
         out["served"] = bench_served(ctx, n_requests=a.served_requests, log=log)
+    if a.copy_probe:
+        # a device-to-device copy of known size: its HBM counters per dispatch calibrate the
+        # traversal's (FETCH_SIZE / WRITE_SIZE runs of a profiler over this program)
+        x = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = x.numel() * 4
     print(json.dumps(out))
 
 

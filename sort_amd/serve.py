@@ -241,7 +241,7 @@ def _pct(x, q):
     return float(np.percentile(np.asarray(x), q))
 
 
-def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, log=print):
+def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, inflight_waves=0, graphs=False, log=print):
     """Served workload, both ways. Returns a dict of p50/p99 latency and throughput for the
     Poisson-arrival run and for a saturated run (every request submitted at once), served versus
     one request at a time. Images are not compared here (the tests do that)."""
@@ -252,7 +252,7 @@ def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, log=pri
         f"{time.perf_counter() - t:.1f}s")
     trace = request_trace(n_requests, rate=rate)
     paths = sum(r["width"] * r["height"] * r["spp"] for r in trace)
-    srv = Server(ctx, wave_paths=wave_paths)
+    srv = Server(ctx, wave_paths=wave_paths, inflight_waves=inflight_waves, graphs=graphs)
     ids = add_scenes(srv, built)
     # warm-up: load every scene and run one tiny request on each
     for i in ids:
@@ -286,6 +286,7 @@ def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, log=pri
         log(f"[served] {name}: one-at-a-time {o}")
     for pt in pts:
         pt.close()
+    res["server"] = {"wave_paths": wave_paths or (1 << 22), "inflight_waves": inflight_waves or 2, "graphs": graphs}
     res["requests"] = n_requests
     res["rate_per_s"] = rate
     res["paths"] = paths
