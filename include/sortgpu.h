@@ -127,12 +127,17 @@ int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint
 
 /* 0 in any field keeps the default. refill = idle lanes of a wave that trigger taking new rays
  * (default 16, 1..64); lds_entries = traversal-stack entries per lane kept in LDS (8 or 16,
- * default 16); blocks_per_cu = cap on resident workgroups per CU (default: occupancy limit). */
+ * default 16); blocks_per_cu = cap on resident workgroups per CU (default: occupancy limit);
+ * leaf_batch = lanes of a wave holding a leaf that end a node-walking phase (default 32, 1..64);
+ * leaf_min = a leaf-testing phase ends when fewer lanes hold a leaf (default 1, 1..64). The knobs
+ * change speed only: every hit is the same for every setting. */
 typedef struct sortgpu_tuning {
     uint32_t refill;
     uint32_t lds_entries;
     uint32_t blocks_per_cu;
-    uint32_t reserved;
+    uint32_t leaf_batch;
+    uint32_t leaf_min;
+    uint32_t reserved[3];
 } sortgpu_tuning;
 
 /* Applies to every later query, render and pool trace of the context. */

@@ -56,8 +56,9 @@ class Pool:
             pass
 
 
-def set_tuning(ctx: Context, refill=0, lds_entries=0, blocks_per_cu=0):
-    t = _lib.Tuning(refill, lds_entries, blocks_per_cu, 0)
+def set_tuning(ctx: Context, refill=0, lds_entries=0, blocks_per_cu=0, leaf_batch=0, leaf_min=0):
+    """Traversal knobs of the context (0 = default); they change speed only, never a hit."""
+    t = _lib.Tuning(refill, lds_entries, blocks_per_cu, leaf_batch, leaf_min)
     check(lib().sortgpu_ctx_set_tuning(ctx.handle, ctypes.byref(t)), ctx.handle)
 
 

@@ -45,6 +45,8 @@ void apply_tuning(const sortgpu_ctx* ctx, TraceArgs& a) {
     a.refill = ctx->tuning.refill;
     a.lds_entries = ctx->tuning.lds_entries;
     a.blocks_per_cu = ctx->tuning.blocks_per_cu;
+    a.leaf_batch = ctx->tuning.leaf_batch;
+    a.leaf_min = ctx->tuning.leaf_min;
 }
 
 int ctx_bind(sortgpu_ctx* ctx) {
@@ -109,8 +111,10 @@ int sortgpu_ctx_set_tuning(sortgpu_ctx* ctx, const sortgpu_tuning* t) {
         set_error(ctx, "sortgpu_ctx_set_tuning: null argument");
         return SORTGPU_ERR_ARG;
     }
-    if (t->refill > 64 || (t->lds_entries != 0 && t->lds_entries != 8 && t->lds_entries != 16)) {
-        set_error(ctx, "sortgpu_ctx_set_tuning: refill must be <= 64 and lds_entries 0, 8 or 16");
+    if (t->refill > 64 || (t->lds_entries != 0 && t->lds_entries != 8 && t->lds_entries != 16) || t->leaf_batch > 64 ||
+        t->leaf_min > 64) {
+        set_error(ctx,
+                  "sortgpu_ctx_set_tuning: refill, leaf_batch and leaf_min must be <= 64 and lds_entries 0, 8 or 16");
         return SORTGPU_ERR_ARG;
     }
     ctx->tuning = *t;

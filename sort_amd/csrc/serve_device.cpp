@@ -365,12 +365,14 @@ class HipDevice : public Device {
     // descriptor). Captured on first use per depth, again when a scene changes the tree table's
     // width or stack depth or the context's tuning changes. The graph holds kernel launches only.
     int wave_graph(uint32_t k, uint32_t depth, hipGraphExec_t* out) {
-        const sortgpu_tuning& t = ctx_->tuning;
-        const uint64_t stamp = ((uint64_t)width_ << 56) ^ ((uint64_t)max_stack_ << 40) ^ ((uint64_t)t.refill << 24) ^
-                               ((uint64_t)t.lds_entries << 12) ^ (uint64_t)t.blocks_per_cu;
-        if (stamp != graph_stamp_) {
+        GraphKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.tuning = ctx_->tuning;
+        key.width = width_;
+        key.max_stack = max_stack_;
+        if (std::memcmp(&key, &graph_key_, sizeof(key)) != 0) {
             drop_graphs();
-            graph_stamp_ = stamp;
+            graph_key_ = key;
         }
         auto it = gexec_[k].find(depth);
         if (it != gexec_[k].end()) {
@@ -429,7 +431,11 @@ class HipDevice : public Device {
     bool graphs_;
     WaveDesc* d_desc_[kMaxStreams] = {};
     std::map<uint32_t, hipGraphExec_t> gexec_[kMaxStreams];
-    uint64_t graph_stamp_ = 0;
+    struct GraphKey {  // what a captured wave graph bakes in besides the depth
+        sortgpu_tuning tuning;
+        uint32_t width, max_stack;
+    };
+    GraphKey graph_key_{};
     hipStream_t s_[kMaxStreams] = {};
     hipStream_t s_up_ = nullptr;
     hipEvent_t ev_up_ = nullptr, ev_film_ = nullptr;

@@ -29,6 +29,7 @@ namespace sortgpu {
 
 constexpr int kBlock = 256;
 constexpr int kDefaultLds = 16;
+constexpr uint32_t kDefaultLeafBatch = 32;
 
 template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS>
 __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
@@ -42,11 +43,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
     const uint32_t lane = __lane_id();
     const uint32_t n_rays = a.n_dev ? *a.n_dev : a.n;
     const uint32_t refill = a.refill ? a.refill : 16u;
+    const uint32_t leaf_batch = a.leaf_batch ? a.leaf_batch : kDefaultLeafBatch;
+    const uint32_t leaf_min = a.leaf_min ? a.leaf_min : 1u;
     const NodeRec<W>* __restrict__ nodes = static_cast<const NodeRec<W>*>(a.tree.nodes);
     const PrimRec* __restrict__ prims = a.tree.prims;
     const QuadRec* __restrict__ quads = a.tree.quads;
 
     RayK r;
+    uint32_t li = 0xffffffffu;  // next primitive record of this lane's leaf
     uint32_t ray_id = 0xffffffffu;
     bool more = true;
     int sp = 0;
@@ -68,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
     };
 
     while (true) {
-        const bool idle = sp == 0;
+        const bool idle = sp == 0 && li == 0xffffffffu;
         const uint64_t want = __ballot(idle && more);
         const uint64_t busy = __ballot(!idle);
         if (want != 0 && (busy == 0 || (uint32_t)__popcll(want) >= refill)) {
@@ -125,94 +129,114 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                 }
             }
         }
-        if (__ballot(sp > 0) == 0) {
+        if (__ballot(sp > 0 || li != 0xffffffffu) == 0) {
             if (__ballot(more) == 0) break;
             continue;
         }
-        if (sp > 0) {
-            --sp;
-            uint32_t ref;
-            float tm;
-            if (sp < LDS) {
-                ref = s_ref[sp * kBlock + tid];
-                tm = s_tm[sp * kBlock + tid];
-            } else {
-                const int e = sp - LDS < SPILL ? sp - LDS : 0;
-                ref = sp_ref[e];
-                tm = sp_tm[e];
-            }
-            bool occluded = false;
-            if (!(bt < tm)) {
-                if (ref & REF_LEAF) {
-                    if (STATS) n_leaves++;
-                    uint32_t i = ref & ~REF_LEAF;
-                    while (true) {
-                        const PrimRec pr = load_prim(prims, i);
-                        float t, u, v;
-                        bool hit = false;
-                        if (pr.flags & PR_QUAD) {
-                            if (STATS) n_quads++;
-                            if (quad_test(r, quads[pr.aux], bt, t)) {
-                                bt = t;
-                                bp = pr.prim;
-                                bu = bv = 0.f;
-                                hit = true;
-                            }
-                        } else {
-                            if (STATS) n_tris++;
-                            if (tri_test(r, pr, t, u, v) && t < bt) {
-                                bt = t;
-                                bp = pr.prim;
-                                bu = u;
-                                bv = v;
-                                hit = true;
-                            }
-                        }
-                        if (ANY && hit) {
-                            occluded = true;
-                            break;
-                        }
-                        if (pr.flags & PR_LAST) break;
-                        ++i;
-                    }
+        // Node phase: lanes without a leaf walk inner nodes (pop, prune, test the children, push)
+        // until they pop a leaf or their stack empties; it ends when no lane can walk or
+        // `leaf_batch` lanes hold a leaf, so node steps of many lanes run together.
+        while (true) {
+            const bool walk = li == 0xffffffffu && sp > 0;
+            if (__ballot(walk) == 0 || (uint32_t)__popcll(__ballot(li != 0xffffffffu)) >= leaf_batch) break;
+            if (walk) {
+                --sp;
+                uint32_t ref;
+                float tm;
+                if (sp < LDS) {
+                    ref = s_ref[sp * kBlock + tid];
+                    tm = s_tm[sp * kBlock + tid];
                 } else {
-                    if (STATS) n_nodes++;
-                    float f[W];
-                    uint32_t ch[W];
-                    const uint32_t m = node_test<W>(r, nodes, ref, f, ch);
-                    const int nh = __popc(m);
-                    uint32_t elig = m;
-                    if (nh >= 3) {
-                        // the >=3 branch pushes by repeated argmax over f > -1 (fast_bvh.hpp:438-453)
+                    const int e = sp - LDS < SPILL ? sp - LDS : 0;
+                    ref = sp_ref[e];
+                    tm = sp_tm[e];
+                }
+                if (!(bt < tm)) {
+                    if (ref & REF_LEAF) {
+                        if (STATS) n_leaves++;
+                        li = ref & ~REF_LEAF;
+                    } else {
+                        if (STATS) n_nodes++;
+                        float f[W];
+                        uint32_t ch[W];
+                        const uint32_t m = node_test<W>(r, nodes, ref, f, ch);
+                        const int nh = __popc(m);
+                        uint32_t elig = m;
+                        if (nh >= 3) {
+                            // the >=3 branch pushes by repeated argmax over f > -1 (fast_bvh.hpp:438-453)
 #pragma unroll
-                        for (int j = 0; j < W; j++)
-                            if (!(f[j] > -1.0f)) elig &= ~(1u << j);
-                    }
-#pragma unroll
-                    for (int j = 0; j < W; j++) {
-                        if (elig & (1u << j)) {
-                            int rank = 0;
-#pragma unroll
-                            for (int k = 0; k < W; k++) {
-                                if (k == j) continue;
-                                const bool before = (f[k] > f[j]) | ((f[k] == f[j]) & (k < j));
-                                rank += ((elig >> k) & 1u) & (before ? 1u : 0u);
-                            }
-                            push(sp + rank, ch[j], f[j]);
+                            for (int j = 0; j < W; j++)
+                                if (!(f[j] > -1.0f)) elig &= ~(1u << j);
                         }
+#pragma unroll
+                        for (int j = 0; j < W; j++) {
+                            if (elig & (1u << j)) {
+                                int rank = 0;
+#pragma unroll
+                                for (int k = 0; k < W; k++) {
+                                    if (k == j) continue;
+                                    const bool before = (f[k] > f[j]) | ((f[k] == f[j]) & (k < j));
+                                    rank += ((elig >> k) & 1u) & (before ? 1u : 0u);
+                                }
+                                push(sp + rank, ch[j], f[j]);
+                            }
+                        }
+                        sp += __popc(elig);
+                        // Only reachable on overflow (the host sizes the stack for the tree): every
+                        // slot below CAP was written by this push, so popping stays defined.
+                        if (sp > CAP) sp = CAP;
                     }
-                    sp += __popc(elig);
-                    // Only reachable on overflow (the host sizes the stack for the tree): every
-                    // slot below CAP was written by this push, so popping stays defined.
-                    if (sp > CAP) sp = CAP;
                 }
             }
-            if (occluded) sp = 0;
-            if (sp == 0) {
-                if (ANY) static_cast<uint8_t*>(a.out)[ray_id] = occluded ? 1 : 0;
-                else reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);
-                ray_id = 0xffffffffu;
+        }
+        // Leaf phase: the lanes holding a leaf test its primitive records together, one record
+        // per step, in record order (a lane keeps its place in an unfinished leaf for the next
+        // round and walks no node before finishing it). It may stop while leaves are unfinished
+        // only when fewer than `leaf_min` lanes hold one and the next node phase can take a step
+        // (some lane can walk and fewer than `leaf_batch` lanes hold a leaf), so every round
+        // advances some lane whatever the knobs.
+        bool occluded = false;
+        while (true) {
+            const uint64_t holding = __ballot(li != 0xffffffffu);
+            if (holding == 0) break;
+            const uint32_t cnt = (uint32_t)__popcll(holding);
+            if (cnt < leaf_min && cnt < leaf_batch && __ballot(li == 0xffffffffu && sp > 0) != 0) break;
+            if (li != 0xffffffffu) {
+                const PrimRec pr = load_prim(prims, li);
+                float t, u, v;
+                bool hit = false;
+                if (pr.flags & PR_QUAD) {
+                    if (STATS) n_quads++;
+                    if (quad_test(r, quads[pr.aux], bt, t)) {
+                        bt = t;
+                        bp = pr.prim;
+                        bu = bv = 0.f;
+                        hit = true;
+                    }
+                } else {
+                    if (STATS) n_tris++;
+                    if (tri_test(r, pr, t, u, v) && t < bt) {
+                        bt = t;
+                        bp = pr.prim;
+                        bu = u;
+                        bv = v;
+                        hit = true;
+                    }
+                }
+                if (ANY && hit) {
+                    occluded = true;
+                    li = 0xffffffffu;
+                    sp = 0;
+                } else {
+                    li = (pr.flags & PR_LAST) ? 0xffffffffu : li + 1;
+                }
             }
+        }
+        if (occluded) sp = 0;
+        if (ray_id != 0xffffffffu && sp == 0 && li == 0xffffffffu) {
+            if (ANY) static_cast<uint8_t*>(a.out)[ray_id] = occluded ? 1 : 0;
+            else reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);
+            ray_id = 0xffffffffu;
         }
     }
 

@@ -26,6 +26,8 @@ struct TraceArgs {
     uint32_t refill;               // idle lanes that trigger a refill (default 16)
     uint32_t lds_entries;          // LDS stack entries per lane: 8 or 16 (default 16)
     uint32_t blocks_per_cu;        // cap on resident workgroups per CU (default: occupancy limit)
+    uint32_t leaf_batch;           // lanes holding a leaf that end a node phase (default 32)
+    uint32_t leaf_min;             // a leaf phase ends below this many lanes holding a leaf (default 1)
 };
 
 // Launches the closest (any = false) or any-hit traversal on `stream`.

@@ -70,11 +70,15 @@ def test_tunings_and_widths_give_identical_hits(gpu_ctx, work8):
     d_sid = torch.from_numpy(w["scene_of_ray"].view(np.int32)).cuda()
     pool = batch.Pool(gpu_ctx, w["accels"])
     ref = pool.trace(d_rays, d_sid).cpu().numpy()
+    ref_any = pool.trace(d_rays, d_sid, any_hit=True).cpu().numpy()
     try:
-        for refill, lds, bpc in [(8, 8, 0), (32, 16, 2), (64, 8, 1), (1, 16, 0)]:
-            batch.set_tuning(gpu_ctx, refill, lds, bpc)
+        for refill, lds, bpc, lb, lm in [(8, 8, 0, 0, 0), (32, 16, 2, 64, 1), (64, 8, 1, 1, 1), (1, 16, 0, 16, 8),
+                                         (16, 16, 0, 48, 32), (16, 8, 0, 64, 64)]:
+            batch.set_tuning(gpu_ctx, refill, lds, bpc, lb, lm)
             got = pool.trace(d_rays, d_sid).cpu().numpy()
-            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (refill, lds, bpc)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (refill, lds, bpc, lb, lm)
+            occ = pool.trace(d_rays, d_sid, any_hit=True).cpu().numpy()
+            assert np.array_equal(occ, ref_any), (refill, lds, bpc, lb, lm)
     finally:
         batch.set_tuning(gpu_ctx, 0, 0, 0)
     # 4-wide trees over the same rays: the per-triangle arithmetic does not depend on the tree,
