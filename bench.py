@@ -31,6 +31,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -60,6 +61,23 @@ def parse():
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+TRAFFIC_PROFILE = "profiles/k_trace_obvh_1m_soup_incoherent.md"
+
+
+def profiled_traffic(a):
+    """Memory-side bytes per incoherent launch, measured by FETCH_SIZE and WRITE_SIZE profiler
+    passes over this workload and calibrated on a device copy of known size, as recorded in the
+    profile summary; None when the workload differs from the profiled one."""
+    if a.tris != 1 << 20 or a.width != 8:
+        return None
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            m = re.search(r"^traffic_bytes_per_launch: (\d+)$", f.read(), re.M)
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
 
 
 def leg_pt(ctx, cpu=True):
@@ -98,13 +116,18 @@ def leg_pt(ctx, cpu=True):
         if cpu:
             from oracle.oracle import OraclePT
             o = OraclePT(sc)
-            crop_rows = H if name == "cornell" else 32
+            # the whole Cornell frame; for the spheres scene a band of rows through the middle of
+            # the frame (ground, spheres and sky in the proportions of the full image's centre)
+            rows = H if name == "cornell" else 64
+            row0 = (H - rows) // 2
             ospp = spp if name == "cornell" else 4
             st = {}
-            o.render(sc["camera"], W, H, ospp, depth, 1, threads=threads, crop=(0, W * crop_rows), stats=st)
-            cpaths = W * crop_rows * ospp
+            o.render(sc["camera"], W, H, ospp, depth, 1, threads=threads, crop=(W * row0, W * (row0 + rows)),
+                     stats=st)
+            cpaths = W * rows * ospp
             res["cpu_baseline"] = {"msamples_s": round(cpaths / st["seconds"] / 1e6, 3), "cores": threads,
-                                   "kind": "port", "sample": f"{W}x{crop_rows} pixels at {ospp} spp, depth {depth}"}
+                                   "kind": "port",
+                                   "sample": f"rows {row0}..{row0 + rows - 1} of {W}x{H} at {ospp} spp, depth {depth}"}
             res["gpu_over_cpu"] = round(res["msamples_s"] / res["cpu_baseline"]["msamples_s"], 1)
         log(f"[pt] {name}: {res}")
         out[name] = res
@@ -262,7 +285,8 @@ def main():
                    "bvh_build_ms": info["build_ms"], "bvh_depth": info["depth"],
                    "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(a),
+                     "traffic_source": TRAFFIC_PROFILE if profiled_traffic(a) else None,
                      "bytes_per_ray": round(b_inc, 1), "launch_ms_mean": round(mean_ms, 4),
                      "launch_ms_median": round(med_ms, 4),
                      "visits_per_ray": {"inner_nodes": round(st_inc["inner_nodes"] / n, 3),

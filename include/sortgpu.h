@@ -232,9 +232,9 @@ typedef struct sortgpu_server sortgpu_server;
 
 typedef struct sortgpu_server_opts {
     uint64_t hbm_budget;     /* bytes the server may hold: scenes + films + workspace (0 = 96 GiB) */
-    uint32_t wave_paths;     /* path slots per wave (0 = 4,194,304) */
+    uint32_t wave_paths;     /* path slots per wave (0 = 8,388,608; 254 B of workspace each) */
     uint32_t max_active;     /* requests admitted at once, 1..32 (0 = 32) */
-    uint32_t inflight_waves; /* waves in flight at once, 1..4 (0 = 2), each on a stream and path
+    uint32_t inflight_waves; /* waves in flight at once, 1..4 (0 = 3), each on a stream and path
                                 workspace of its own, so their bounce loops overlap */
     uint32_t manual;         /* 1: no worker thread; the caller drives sortgpu_server_step */
     uint32_t host_device;    /* 1: host-only stand-in device (no GPU; films hold a hash of the

@@ -286,7 +286,7 @@ def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, infligh
         log(f"[served] {name}: one-at-a-time {o}")
     for pt in pts:
         pt.close()
-    res["server"] = {"wave_paths": wave_paths or (1 << 22), "inflight_waves": inflight_waves or 2, "graphs": graphs}
+    res["server"] = {"wave_paths": wave_paths or (1 << 23), "inflight_waves": inflight_waves or 3, "graphs": graphs}
     res["requests"] = n_requests
     res["rate_per_s"] = rate
     res["paths"] = paths
