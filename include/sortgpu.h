@@ -213,6 +213,16 @@ void sortgpu_scene_destroy(sortgpu_scene* scene);
 int sortgpu_render(sortgpu_scene* scene, const sortgpu_camera* camera, uint32_t width, uint32_t height,
                    uint32_t spp, uint32_t max_depth, uint64_t seed, float* d_film_rgb, void* stream);
 
+/* The same frame as sortgpu_render, recording every extension ray PathTracing::li traces
+ * (pathtracing.cpp:44-258; bounce 0 = the camera ray) with its closest hit, for the paths of pixels
+ * [pix_begin, pix_end): slot ((pixel - pix_begin) * spp + sample) * max_depth + bounce of d_rays,
+ * d_hits and d_valid (1 where the path traced that bounce; cleared by the call). Buffers hold
+ * (pix_end - pix_begin) * spp * max_depth entries (device). A test and debugging aid: the
+ * reference has no such entry point. */
+int sortgpu_render_ray_dump(sortgpu_scene* scene, const sortgpu_camera* camera, uint32_t width, uint32_t height,
+                            uint32_t spp, uint32_t max_depth, uint64_t seed, uint32_t pix_begin, uint32_t pix_end,
+                            sortgpu_ray* d_rays, sortgpu_hit* d_hits, uint8_t* d_valid, void* stream);
+
 /* Batched Integrator::Li (integrator.h:56): radiance along n caller rays. d_keys holds per ray
  * (pixel, sample) — the random-number key of the path. d_rgb: n x 3 floats (device). */
 int sortgpu_li(sortgpu_scene* scene, const sortgpu_ray* d_rays, const uint32_t* d_keys, uint32_t n,

@@ -228,6 +228,9 @@ def _pt_sigs():
     l.oracle_pt_render.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _P, _P]
     l.oracle_pt_li.argtypes = [_P, _P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _P]
+    l.oracle_pt_dump.restype = ctypes.c_double
+    l.oracle_pt_dump.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _P, _P, _P]
     l.oracle_bxdf.argtypes = [_P, ctypes.c_int, ctypes.c_uint32, _P, _P, _P, _P]
     l._pt_ready = True
     return l
@@ -298,6 +301,20 @@ class OraclePT:
         if stats is not None:
             stats.update(extension_rays=int(st[0]), shadow_rays=int(st[1]), seconds=secs)
         return out
+
+    def ray_dump(self, cam, width, height, spp, max_depth, seed, pix0, pix1, threads=8):
+        """Every extension ray (bounce 0 = camera ray) of the paths of pixels [pix0, pix1) with its
+        closest hit, at slot ((pixel - pix0) * spp + sample) * max_depth + bounce. Returns
+        (rays (n, 8) float32, hits (n, 4) float32 with prim as uint32 bits, valid (n,) bool)."""
+        import math
+        c = np.array([*cam["eye"], *cam["target"], *cam["up"], math.radians(cam["fov_y_deg"])], np.float32)
+        n = (pix1 - pix0) * spp * max_depth
+        rays = np.zeros((n, 8), np.float32)
+        hits = np.zeros((n, 4), np.float32)
+        valid = np.zeros(n, np.uint8)
+        _pt_sigs().oracle_pt_dump(self.h, c.ctypes.data, width, height, pix0, pix1, spp, max_depth, seed, threads,
+                                  rays.ctypes.data, hits.ctypes.data, valid.ctypes.data)
+        return rays, hits, valid.astype(bool)
 
     def li(self, rays, keys, max_depth, seed, threads=8):
         rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 8))

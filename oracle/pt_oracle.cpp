@@ -781,9 +781,17 @@ static Spectrum EvaluateDirect(const PtScene& sc, const ScatteringEvent& se, Vec
     return radiance;
 }
 
+// Where li records its extension rays and their hits (slot = base + bounce), when asked to.
+struct RayDump {
+    RayRec* rays;
+    HitRec* hits;
+    uint8_t* valid;
+    size_t base;
+};
+
 // PathTracing::li for one camera (or caller) ray.
 static Spectrum li(const PtScene& sc, Vec ro, Vec rd, float rtmin, uint32_t seed, uint32_t pixel, uint32_t sample,
-                   uint32_t max_depth, uint64_t& ext_rays, uint64_t& shadow_rays) {
+                   uint32_t max_depth, uint64_t& ext_rays, uint64_t& shadow_rays, const RayDump* dump = nullptr) {
     Spectrum L(0.0f), throughput(1.0f);
     uint32_t bounces = 0;
     Vec o = ro, d = rd;
@@ -793,7 +801,14 @@ static Spectrum li(const PtScene& sc, Vec ro, Vec rd, float rtmin, uint32_t seed
         if (bounces >= max_depth) return L;
         HitRec h;
         ext_rays++;
-        if (!closest(sc, o, d, tmin, FLT_MAX, h)) {
+        const bool hit_any = closest(sc, o, d, tmin, FLT_MAX, h);
+        if (dump) {
+            const size_t k = dump->base + bounces;
+            dump->rays[k] = RayRec{o.x, o.y, o.z, tmin, d.x, d.y, d.z, FLT_MAX};
+            dump->hits[k] = h;
+            dump->valid[k] = 1;
+        }
+        if (!hit_any) {
             if (bounces == 0) return sc.has_env ? sc.env : Spectrum(0.0f);
             break;
         }
@@ -891,11 +906,35 @@ void* oracle_pt_create(OracleHandle* bvh, const float* pos, uint32_t nverts, con
 
 void oracle_pt_destroy(void* p) { delete static_cast<PtScene*>(p); }
 
+static double render_impl(void* p, const float* cam10, uint32_t width, uint32_t height, uint32_t pix0,
+                          uint32_t pix1, uint32_t spp, uint32_t max_depth, uint64_t seed64, int nthreads, float* out,
+                          uint64_t* stats, RayRec* dump_rays, HitRec* dump_hits, uint8_t* dump_valid);
+
 // Renders with the pixel-major loop of ImageEvaluation (each pixel: spp samples, mean of the
 // valid ones). pixel range [pix0, pix1) only (a crop); out: (pix1 - pix0) x 3.
 // stats[0..1] += extension rays, shadow rays. Returns seconds.
 double oracle_pt_render(void* p, const float* cam10, uint32_t width, uint32_t height, uint32_t pix0, uint32_t pix1,
                         uint32_t spp, uint32_t max_depth, uint64_t seed64, int nthreads, float* out, uint64_t* stats) {
+    return render_impl(p, cam10, width, height, pix0, pix1, spp, max_depth, seed64, nthreads, out, stats, nullptr,
+                       nullptr, nullptr);
+}
+
+// The same render, also recording every extension ray (the camera ray is bounce 0) with its
+// closest hit: slot ((pixel - pix0) * spp + sample) * max_depth + bounce of rays8 (8 floats),
+// hits4 (t, prim, u, v) and valid (1 where the path traced that bounce; the caller zeroes it).
+double oracle_pt_dump(void* p, const float* cam10, uint32_t width, uint32_t height, uint32_t pix0, uint32_t pix1,
+                      uint32_t spp, uint32_t max_depth, uint64_t seed64, int nthreads, float* rays8, float* hits4,
+                      uint8_t* valid) {
+    std::vector<float> out((size_t)(pix1 - pix0) * 3);
+    return render_impl(p, cam10, width, height, pix0, pix1, spp, max_depth, seed64, nthreads, out.data(), nullptr,
+                       reinterpret_cast<RayRec*>(rays8), reinterpret_cast<HitRec*>(hits4), valid);
+}
+
+}  // extern "C"
+
+static double render_impl(void* p, const float* cam10, uint32_t width, uint32_t height, uint32_t pix0,
+                          uint32_t pix1, uint32_t spp, uint32_t max_depth, uint64_t seed64, int nthreads, float* out,
+                          uint64_t* stats, RayRec* dump_rays, HitRec* dump_hits, uint8_t* dump_valid) {
     const PtScene& sc = *static_cast<PtScene*>(p);
     const uint32_t seed = (uint32_t)(seed64 ^ (seed64 >> 32));
     // camera (perspective.cpp:26-104): ViewLookat basis and Perspective scales
@@ -926,7 +965,8 @@ double oracle_pt_render(void* p, const float* cam10, uint32_t width, uint32_t he
                     const float cy = 1.0f - 2.0f * ry / (float)height;
                     const Vec dc = normalize(Vec(cx / xScale, cy / yScale, 1.0f));
                     const Vec d = right * dc.x + up2 * dc.y + fwd * dc.z;
-                    const Spectrum l = li(sc, eye, d, 0.0f, seed, pix, k, max_depth, e, s);
+                    RayDump dd{dump_rays, dump_hits, dump_valid, ((size_t)(pix - pix0) * spp + k) * max_depth};
+                    const Spectrum l = li(sc, eye, d, 0.0f, seed, pix, k, max_depth, e, s, dump_rays ? &dd : nullptr);
                     if (l.IsValid()) radiance = radiance + l;
                     else --valid;
                 }
@@ -950,6 +990,8 @@ double oracle_pt_render(void* p, const float* cam10, uint32_t width, uint32_t he
     }
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+
+extern "C" {
 
 // Batched Li over caller rays: keys = (pixel, sample) per ray; out n x 3.
 void oracle_pt_li(void* p, const float* rays8, const uint32_t* keys, uint32_t n, uint32_t max_depth, uint64_t seed64,

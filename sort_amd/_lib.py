@@ -110,6 +110,8 @@ _SIGS = {
     "sortgpu_scene_destroy": (None, [_P]),
     "sortgpu_render": (_I, [_P, ctypes.POINTER(Camera), _U32, _U32, _U32, _U32, ctypes.c_uint64, _P, _P]),
     "sortgpu_li": (_I, [_P, _P, _P, _U32, _U32, ctypes.c_uint64, _P, _P]),
+    "sortgpu_render_ray_dump": (_I, [_P, ctypes.POINTER(Camera), _U32, _U32, _U32, _U32, ctypes.c_uint64, _U32, _U32,
+                                     _P, _P, _P, _P]),
     "sortgpu_render_stats": (_I, [_P, _P]),
     "sortgpu_server_create": (_I, [_P, ctypes.POINTER(ServerOpts), ctypes.POINTER(_P)]),
     "sortgpu_server_destroy": (None, [_P]),

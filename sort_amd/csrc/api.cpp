@@ -49,6 +49,14 @@ void apply_tuning(const sortgpu_ctx* ctx, TraceArgs& a) {
     a.leaf_min = ctx->tuning.leaf_min;
 }
 
+void ctx_acquire(sortgpu_ctx* ctx) {
+    if (ctx) ctx->refs.fetch_add(1);
+}
+
+void ctx_release(sortgpu_ctx* ctx) {
+    if (ctx && ctx->refs.fetch_sub(1) == 1) delete ctx;
+}
+
 int ctx_bind(sortgpu_ctx* ctx) {
     int cur = -1;
     hipError_t e = hipGetDevice(&cur);
@@ -104,7 +112,9 @@ int sortgpu_ctx_create(int device, sortgpu_ctx** out) {
     return SORTGPU_OK;
 }
 
-void sortgpu_ctx_destroy(sortgpu_ctx* ctx) { delete ctx; }
+// Drops the creator's reference; the context lives on while accelerators, pools, scenes or
+// servers made from it do.
+void sortgpu_ctx_destroy(sortgpu_ctx* ctx) { ctx_release(ctx); }
 
 int sortgpu_ctx_set_tuning(sortgpu_ctx* ctx, const sortgpu_tuning* t) {
     if (!ctx || !t) {
@@ -196,14 +206,17 @@ int sortgpu_accel_build(sortgpu_ctx* ctx, const float* pos, uint32_t nverts, con
             return rc;
         }
     }
+    ctx_acquire(ctx);
     *out = acc;
     return SORTGPU_OK;
 }
 
 void sortgpu_accel_destroy(sortgpu_accel* acc) {
     if (!acc) return;
+    sortgpu_ctx* ctx = acc->ctx;
     free_accel_device(acc);
     delete acc;
+    ctx_release(ctx);
 }
 
 int sortgpu_accel_valid(const sortgpu_accel* acc) { return acc && acc->host.valid ? 1 : 0; }

@@ -20,6 +20,7 @@ struct sortgpu_ctx {
     sortgpu_tuning tuning{};
     std::mutex mu;
     std::string err;
+    std::atomic<int> refs{1};  // the creator's, plus one per object made from the context
 };
 
 namespace sortgpu {
@@ -51,6 +52,9 @@ namespace sortgpu {
 void set_error(sortgpu_ctx* ctx, const char* fmt, ...);
 int hip_fail(sortgpu_ctx* ctx, hipError_t e, const char* what);
 int ctx_bind(sortgpu_ctx* ctx);
+// Reference counting of contexts: every object made from a context holds one reference.
+void ctx_acquire(sortgpu_ctx* ctx);
+void ctx_release(sortgpu_ctx* ctx);
 int upload_accel(sortgpu_accel* acc);
 // Copies the context's tuning knobs into a launch.
 void apply_tuning(const sortgpu_ctx* ctx, struct TraceArgs& a);

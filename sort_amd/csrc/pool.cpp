@@ -70,6 +70,7 @@ int sortgpu_pool_create(sortgpu_ctx* ctx, sortgpu_accel* const* accels, uint32_t
     sortgpu_pool* p = new (std::nothrow) sortgpu_pool();
     if (!p) return SORTGPU_ERR_OOM;
     p->ctx = ctx;
+    ctx_acquire(ctx);
     p->n = n;
     p->width = width;
     p->max_stack = max_stack;
@@ -129,7 +130,9 @@ void sortgpu_pool_destroy(sortgpu_pool* p) {
     void* ptrs[] = {p->d_block, p->d_views, p->d_counters, p->d_error};
     for (void* x : ptrs)
         if (x) hipFree(x);
+    sortgpu_ctx* ctx = p->ctx;
     delete p;
+    ctx_release(ctx);
 }
 
 int sortgpu_pool_info(const sortgpu_pool* p, uint64_t out[4]) {

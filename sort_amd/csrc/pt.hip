@@ -130,6 +130,22 @@ __global__ void k_zero_counts(uint32_t* count, int which) {
     }
 }
 
+// Records the extension rays of the current queue and their hits into the ray dump (paths of
+// pixels in the dump's range only).
+__global__ __launch_bounds__(kPtBlock) void k_dump(uint32_t bounce, int cur, PathState ps, Queues q, RayDumpDev d) {
+    const uint32_t n = q.count[cur];
+    for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
+        const uint32_t p = q.pid[cur][i];
+        const uint32_t pix = ps.key[p], s = ps.smp[p];
+        if (pix < d.pix_begin || pix >= d.pix_end || s >= d.spp || bounce >= d.depth) continue;
+        const size_t k = ((size_t)(pix - d.pix_begin) * d.spp + s) * d.depth + bounce;
+        d.rays[2 * k] = q.ray[cur][2 * i];
+        d.rays[2 * k + 1] = q.ray[cur][2 * i + 1];
+        d.hits[k] = q.hit[i];
+        d.valid[k] = 1;
+    }
+}
+
 __global__ void k_add_totals(const uint32_t* count, int ext_slot, unsigned long long* totals) {
     if (threadIdx.x == 0) {
         totals[0] += count[ext_slot];
@@ -457,6 +473,7 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
             ta.max_stack = L.tree_max_stack;
         }
         if ((e = launch_trace(ta, false, false, s, L.num_cus)) != hipSuccess) return e;
+        if (L.dump.rays) hipLaunchKernelGGL(k_dump, dim3(shade_blocks), dim3(kPtBlock), 0, s, b, cur, L.ps, L.q, L.dump);
         hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, L.q.count, nxt);
         hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.wave->reqs, b, cur, L.ps,
                            L.q);

@@ -13,6 +13,7 @@ using namespace sortgpu;
 
 struct sortgpu_scene {
     sortgpu_accel* accel = nullptr;
+    sortgpu_ctx* ctx = nullptr;   // held (reference counted) until the scene is destroyed
     SceneDev host{};              // pointers are device addresses
     SceneDev* d_scene = nullptr;  // one-entry scene table
     DeviceAllocs mem;             // shading data + scene table
@@ -88,6 +89,8 @@ int sortgpu_scene_create(sortgpu_accel* acc, const sortgpu_scene_desc* desc, sor
     sortgpu_scene* sc = new (std::nothrow) sortgpu_scene();
     if (!sc) return SORTGPU_ERR_OOM;
     sc->accel = acc;
+    sc->ctx = ctx;
+    ctx_acquire(ctx);
     SceneImage im;
     void* block = nullptr;
     void* dsc = nullptr;
@@ -119,7 +122,8 @@ int sortgpu_scene_create(sortgpu_accel* acc, const sortgpu_scene_desc* desc, sor
 
 void sortgpu_scene_destroy(sortgpu_scene* sc) {
     if (!sc) return;
-    ctx_bind(sc->accel->ctx);
+    sortgpu_ctx* ctx = sc->ctx;
+    ctx_bind(ctx);
     sc->ring.release();
     if (sc->ev1) hipEventSynchronize(sc->ev1);
     sc->ws.release();
@@ -128,19 +132,17 @@ void sortgpu_scene_destroy(sortgpu_scene* sc) {
     if (sc->ev0) hipEventDestroy(sc->ev0);
     if (sc->ev1) hipEventDestroy(sc->ev1);
     delete sc;
+    ctx_release(ctx);
 }
 
-int sortgpu_render(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, uint32_t height, uint32_t spp,
-                   uint32_t max_depth, uint64_t seed, float* d_film_rgb, void* stream) {
-    if (!sc || !cam || !d_film_rgb || width == 0 || height == 0 || spp == 0) {
-        set_error(sc ? sc->accel->ctx : nullptr, "sortgpu_render: bad argument");
-        return SORTGPU_ERR_ARG;
-    }
+}  // extern "C"
+
+namespace {
+
+// The frame loop of sortgpu_render; with a dump, also records the extension rays and hits.
+int render_impl(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, uint32_t height, uint32_t spp,
+                uint32_t max_depth, uint64_t seed, float* d_film_rgb, const RayDumpDev* dump, void* stream) {
     sortgpu_ctx* ctx = sc->accel->ctx;
-    if ((uint64_t)width * height > (1ull << 24)) {
-        set_error(ctx, "sortgpu_render: image too large (at most 2^24 pixels)");
-        return SORTGPU_ERR_ARG;
-    }
     int rc = ctx_bind(ctx);
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -172,13 +174,63 @@ int sortgpu_render(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width,
         PtLaunch L = make_launch(sc, d_wave, max_depth);
         L.paths_cap = npix * ns;
         L.pix_cap = npix;
+        if (dump) L.dump = *dump;
         if ((e = pt_wave(L, s)) != hipSuccess) return hip_fail(ctx, e, "render wave");
         if ((rc = sc->ring.done(ctx, s))) return rc;
     }
-    if ((e = pt_resolve_film(sc->film, npix, d_film_rgb, s)) != hipSuccess) return hip_fail(ctx, e, "resolve film");
+    if (d_film_rgb && (e = pt_resolve_film(sc->film, npix, d_film_rgb, s)) != hipSuccess)
+        return hip_fail(ctx, e, "resolve film");
     if ((e = hipEventRecord(sc->ev1, s)) != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
     sc->last_paths = (uint64_t)npix * spp;
     return SORTGPU_OK;
+}
+
+int check_frame(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, uint32_t height, uint32_t spp,
+                const char* fn) {
+    if (!sc || !cam || width == 0 || height == 0 || spp == 0) {
+        set_error(sc ? sc->accel->ctx : nullptr, "%s: bad argument", fn);
+        return SORTGPU_ERR_ARG;
+    }
+    if ((uint64_t)width * height > (1ull << 24)) {
+        set_error(sc->accel->ctx, "%s: image too large (at most 2^24 pixels)", fn);
+        return SORTGPU_ERR_ARG;
+    }
+    return SORTGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sortgpu_render(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, uint32_t height, uint32_t spp,
+                   uint32_t max_depth, uint64_t seed, float* d_film_rgb, void* stream) {
+    int rc = check_frame(sc, cam, width, height, spp, "sortgpu_render");
+    if (rc) return rc;
+    if (!d_film_rgb) {
+        set_error(sc->accel->ctx, "sortgpu_render: d_film_rgb is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    return render_impl(sc, cam, width, height, spp, max_depth, seed, d_film_rgb, nullptr, stream);
+}
+
+int sortgpu_render_ray_dump(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, uint32_t height,
+                            uint32_t spp, uint32_t max_depth, uint64_t seed, uint32_t pix_begin, uint32_t pix_end,
+                            sortgpu_ray* d_rays, sortgpu_hit* d_hits, uint8_t* d_valid, void* stream) {
+    int rc = check_frame(sc, cam, width, height, spp, "sortgpu_render_ray_dump");
+    if (rc) return rc;
+    sortgpu_ctx* ctx = sc->accel->ctx;
+    if (!d_rays || !d_hits || !d_valid || pix_begin >= pix_end || pix_end > width * height || max_depth == 0 ||
+        (uint64_t)(pix_end - pix_begin) * spp * max_depth > (1ull << 31)) {
+        set_error(ctx, "sortgpu_render_ray_dump: null buffer, empty or out-of-frame pixel range, or over 2^31 slots");
+        return SORTGPU_ERR_ARG;
+    }
+    if ((rc = ctx_bind(ctx))) return rc;
+    const size_t n = (size_t)(pix_end - pix_begin) * spp * max_depth;
+    hipError_t e = hipMemsetAsync(d_valid, 0, n, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "clear dump flags");
+    RayDumpDev d{reinterpret_cast<float4*>(d_rays), reinterpret_cast<float4*>(d_hits), d_valid, pix_begin, pix_end,
+                 spp, max_depth};
+    return render_impl(sc, cam, width, height, spp, max_depth, seed, nullptr, &d, stream);
 }
 
 int sortgpu_li(sortgpu_scene* sc, const sortgpu_ray* d_rays, const uint32_t* d_keys, uint32_t n, uint32_t max_depth,

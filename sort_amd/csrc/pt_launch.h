@@ -8,6 +8,15 @@
 
 namespace sortgpu {
 
+// Optional record of every extension ray (bounce 0 = camera ray) with its closest hit, for
+// paths of pixels [pix_begin, pix_end): slot ((pixel - pix_begin) * spp + sample) * depth + bounce.
+struct RayDumpDev {
+    float4* rays;      // 2 float4 per slot, or null: no record
+    float4* hits;      // (t, prim bits, u, v) per slot
+    uint8_t* valid;    // 1 where the path traced that bounce
+    uint32_t pix_begin, pix_end, spp, depth;
+};
+
 struct PtLaunch {
     TraceArgs trace;            // counter / error slots and tuning; the single tree when trees == null
     const SceneDev* scenes;     // device scene table
@@ -22,6 +31,7 @@ struct PtLaunch {
     uint32_t paths_cap;         // grid bound: at least the wave's paths (kernels read the exact count)
     uint32_t pix_cap;           // grid bound: at least the wave's film pixels
     int num_cus;
+    RayDumpDev dump;            // rays == null: none
 };
 
 // Bytes of path workspace per path slot (state + queues), used for HBM budgeting.

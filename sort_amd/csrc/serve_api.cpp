@@ -11,8 +11,12 @@
 using namespace sortgpu;
 
 struct sortgpu_server {
-    sortgpu_ctx* ctx = nullptr;
+    sortgpu_ctx* ctx = nullptr;  // held (reference counted) until the server is destroyed
     std::unique_ptr<serve::Engine> eng;
+    ~sortgpu_server() {
+        eng.reset();
+        ctx_release(ctx);
+    }
 };
 
 namespace {
@@ -60,6 +64,7 @@ int sortgpu_server_create(sortgpu_ctx* ctx, const sortgpu_server_opts* opts, sor
     sortgpu_server* s = new (std::nothrow) sortgpu_server();
     if (!s) return SORTGPU_ERR_OOM;
     s->ctx = ctx;
+    ctx_acquire(ctx);
     s->eng.reset(new (std::nothrow) serve::Engine(std::move(dev), eo));
     if (!s->eng) {
         delete s;

@@ -135,6 +135,23 @@ class PathTracing:
               self.accel.ctx.handle)
         return out
 
+    def ray_dump(self, cam, width, height, spp, pix_begin, pix_end, seed=1, max_depth=None, stream=None):
+        """The extension rays (bounce 0 = camera ray) of the paths of pixels [pix_begin, pix_end)
+        with their closest hits, slot ((pixel - pix_begin) * spp + sample) * max_depth + bounce.
+        Returns device tensors rays (n, 8), hits (n, 4), valid (n,) uint8."""
+        import torch
+        dev = f"cuda:{self.accel.ctx.device}"
+        depth = self.max_recursive_depth if max_depth is None else max_depth
+        n = (pix_end - pix_begin) * spp * depth
+        rays = torch.empty((n, 8), dtype=torch.float32, device=dev)
+        hits = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        valid = torch.empty((n,), dtype=torch.uint8, device=dev)
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device).cuda_stream
+        check(lib().sortgpu_render_ray_dump(self.handle, ctypes.byref(cam), width, height, spp, depth, seed, pix_begin,
+                                            pix_end, rays.data_ptr(), hits.data_ptr(), valid.data_ptr(), s),
+              self.accel.ctx.handle)
+        return rays, hits, valid
+
     def li(self, rays, keys, seed=1, max_depth=None, stream=None):
         """Batched Integrator::Li: rays (n,8) and keys (n,2) = (pixel, sample); returns (n,3)."""
         import torch

@@ -78,3 +78,24 @@ def test_host_build_with_quads_and_options(host_ctx):
                        max_leaf=kw.get("max_pri_in_leaf", 16))
         assert np.array_equal(acc.prim_order(), ob.prim_order()), kw
         assert np.array_equal(acc.tree_dump(), ob.tree_dump()), kw
+
+
+def test_context_outlives_objects_made_from_it():
+    """Destroying a context first is safe: accelerators and servers made from it hold a reference,
+    and the context is freed with the last of them."""
+    from sort_amd import Context, Obvh, scenes
+    from sort_amd.integrator import camera, lambert
+    from sort_amd.serve import Server
+    ctx = Context(-1)
+    pos, idx = scenes.soup(500, seed=3)
+    acc = Obvh(ctx).build(pos, idx)
+    srv = Server(ctx, host_device=True, manual=True, wave_paths=4096)
+    sid = srv.add_scene(acc, [lambert((0.5, 0.5, 0.5))])
+    ctx.close()  # the caller's reference goes first
+    cam = camera((0.5, 0.5, -1.5), (0.5, 0.5, 0.5))
+    rid = srv.submit(sid, cam, 8, 8, 2)
+    srv.drain()
+    assert srv.info(rid)["state"] == "done"
+    assert acc.info()["prims"] == 500
+    srv.close()
+    acc.release()

@@ -222,3 +222,33 @@ def test_quad_light_direct_irradiance_closed_form():
     for (x, z), e in zip(pts, est):
         E = _polygon_irradiance(np.array([x, 0.0, z]), np.array([0, 1, 0]), corners, L)
         assert abs(e - rho / math.pi * E) <= 0.01 * rho / math.pi * E + 1e-4, (x, z, e, rho / math.pi * E)
+
+
+def test_ray_dump_is_consistent_with_the_traversal_and_the_camera():
+    """The oracle's ray dump (every extension ray of PathTracing::li with its closest hit): bounce 0
+    is the camera ray of the pixel's jittered raster point, each recorded hit is what the BVH gives
+    for the recorded ray, a bounce is traced only after the one before it, and the number of
+    recorded rays equals the render's extension-ray count."""
+    sc = scenes.cornell_sphere()
+    pt = O.OraclePT(sc)
+    W = H = 16
+    spp, depth = 4, 5
+    rays, hits, valid = pt.ray_dump(sc["camera"], W, H, spp, depth, 3, 0, W * H, threads=4)
+    st = {}
+    pt.render(sc["camera"], W, H, spp, depth, 3, threads=4, stats=st)
+    assert valid.sum() == st["extension_rays"]
+    v = valid.reshape(W * H, spp, depth)
+    assert v[:, :, 0].all()
+    assert not np.any(v[:, :, 1:] & ~v[:, :, :-1])
+    again = pt.bvh.closest(rays[valid], threads=4)
+    assert np.array_equal(again.view(np.uint32), hits[valid].view(np.uint32))
+    # camera rays start at the eye with tmin 0 and point into the room (+z)
+    r0 = rays.reshape(W * H, spp, depth, 8)[:, :, 0]
+    assert np.allclose(r0[..., 0:3], sc["camera"]["eye"]) and np.all(r0[..., 3] == 0.0)
+    assert np.all(r0[..., 6] > 0.0)
+    # later bounces start where the previous ray hit, with tmin 1e-4
+    r1 = rays.reshape(W * H, spp, depth, 8)[:, :, 1]
+    h0 = hits.reshape(W * H, spp, depth, 4)[:, :, 0]
+    m = v[:, :, 1]
+    p = r0[..., 0:3] + h0[..., 0:1] * r0[..., 4:7]
+    assert np.allclose(r1[m][:, 0:3], p[m], atol=1e-5) and np.all(r1[m][:, 3] == np.float32(1e-4))
