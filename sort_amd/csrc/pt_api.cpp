@@ -17,7 +17,13 @@ struct sortgpu_scene {
     SceneDev host{};              // pointers are device addresses
     SceneDev* d_scene = nullptr;  // one-entry scene table
     DeviceAllocs mem;             // shading data + scene table
-    PathWorkspace ws;
+    // A frame's waves alternate over two internal streams, each with its own workspace, so one
+    // wave's latency-bound last bounces overlap the next wave's first; the caller's stream waits
+    // for both at the end, and film updates run in wave order (events).
+    static constexpr int kStreams = 2;
+    PathWorkspace ws[kStreams];
+    hipStream_t rs[kStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_film = nullptr, ev_join[kStreams] = {};
     DeviceAllocs film_mem;
     float4* film = nullptr;
     uint32_t cap_pix = 0;
@@ -41,7 +47,20 @@ int ensure_film(sortgpu_scene* sc, uint32_t npix) {
     return SORTGPU_OK;
 }
 
-PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_depth) {
+// The internal streams and their events, made on first use.
+int ensure_streams(sortgpu_scene* sc) {
+    if (sc->rs[0]) return SORTGPU_OK;
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < sortgpu_scene::kStreams && e == hipSuccess; k++) {
+        e = hipStreamCreateWithFlags(&sc->rs[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sc->ev_join[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sc->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sc->ev_film, hipEventDisableTiming);
+    return e == hipSuccess ? SORTGPU_OK : hip_fail(sc->ctx, e, "render streams");
+}
+
+PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_depth, int k = 0) {
     PtLaunch L;
     std::memset(&L, 0, sizeof(L));
     sortgpu_accel* acc = sc->accel;
@@ -54,8 +73,8 @@ PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_dep
     L.scenes = sc->d_scene;
     L.wave = d_wave;
     L.trees = nullptr;
-    L.ps = sc->ws.ps;
-    L.q = sc->ws.q;
+    L.ps = sc->ws[k].ps;
+    L.q = sc->ws[k].q;
     L.max_depth = max_depth;
     L.num_cus = acc->ctx->num_cus;
     return L;
@@ -126,7 +145,16 @@ void sortgpu_scene_destroy(sortgpu_scene* sc) {
     ctx_bind(ctx);
     sc->ring.release();
     if (sc->ev1) hipEventSynchronize(sc->ev1);
-    sc->ws.release();
+    for (int k = 0; k < sortgpu_scene::kStreams; k++) {
+        if (sc->rs[k]) {
+            hipStreamSynchronize(sc->rs[k]);
+            hipStreamDestroy(sc->rs[k]);
+        }
+        if (sc->ev_join[k]) hipEventDestroy(sc->ev_join[k]);
+        sc->ws[k].release();
+    }
+    if (sc->ev_fork) hipEventDestroy(sc->ev_fork);
+    if (sc->ev_film) hipEventDestroy(sc->ev_film);
     sc->film_mem.free_all();
     sc->mem.free_all();
     if (sc->ev0) hipEventDestroy(sc->ev0);
@@ -152,7 +180,11 @@ int render_impl(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, ui
     std::memset(&wd, 0, sizeof(wd));
     ReqDev& R = wd.reqs[0];
     if ((rc = make_camera(ctx, cam, width, height, R.cam))) return rc;
-    if ((rc = sc->ws.ensure(ctx, npix * spw)) || (rc = ensure_film(sc, npix))) return rc;
+    const uint32_t waves = (spp + spw - 1) / spw;
+    const int nstreams = waves > 1 ? sortgpu_scene::kStreams : 1;
+    for (int k = 0; k < nstreams; k++)
+        if ((rc = sc->ws[k].ensure(ctx, npix * spw))) return rc;
+    if ((rc = ensure_film(sc, npix)) || (nstreams > 1 && (rc = ensure_streams(sc)))) return rc;
     R.scene = 0;
     R.seed = fold_seed(seed);
     R.max_depth = max_depth;
@@ -161,22 +193,45 @@ int render_impl(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, ui
     hipError_t e = hipEventRecord(sc->ev0, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
     if ((e = pt_zero_f4(sc->film, npix, s)) != hipSuccess) return hip_fail(ctx, e, "zero film");
-    if ((e = pt_zero_f4(reinterpret_cast<float4*>(sc->ws.q.totals), 1, s)) != hipSuccess)
-        return hip_fail(ctx, e, "zero totals");
-    for (uint32_t s0 = 0; s0 < spp; s0 += spw) {
+    for (int k = 0; k < sortgpu_scene::kStreams; k++)
+        if (sc->ws[k].q.totals &&
+            (e = pt_zero_f4(reinterpret_cast<float4*>(sc->ws[k].q.totals), 1, s)) != hipSuccess)
+            return hip_fail(ctx, e, "zero totals");
+    if (nstreams > 1) {  // fork: the internal streams start after the caller's work so far
+        if ((e = hipEventRecord(sc->ev_fork, s)) != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
+        for (int k = 0; k < nstreams; k++)
+            if ((e = hipStreamWaitEvent(sc->rs[k], sc->ev_fork, 0)) != hipSuccess)
+                return hip_fail(ctx, e, "hipStreamWaitEvent");
+    }
+    uint32_t w = 0;
+    for (uint32_t s0 = 0; s0 < spp; s0 += spw, w++) {
         const uint32_t ns = std::min(spw, spp - s0);
+        const int k = (int)(w % (uint32_t)nstreams);
+        hipStream_t st = nstreams > 1 ? sc->rs[k] : s;
         wd.ct.n = 1;
         wd.ct.total_paths = npix * ns;
         wd.ct.total_pix = npix;
         wd.ct.c[0] = ChunkDev{0u, s0, ns, npix, 0u, 0u};
         const WaveDesc* d_wave = nullptr;
-        if ((rc = sc->ring.push(ctx, wd, s, nullptr, &d_wave))) return rc;
-        PtLaunch L = make_launch(sc, d_wave, max_depth);
+        if ((rc = sc->ring.push(ctx, wd, st, nullptr, &d_wave))) return rc;
+        PtLaunch L = make_launch(sc, d_wave, max_depth, k);
         L.paths_cap = npix * ns;
         L.pix_cap = npix;
         if (dump) L.dump = *dump;
-        if ((e = pt_wave(L, s)) != hipSuccess) return hip_fail(ctx, e, "render wave");
-        if ((rc = sc->ring.done(ctx, s))) return rc;
+        if ((e = pt_wave_paths(L, st)) != hipSuccess) return hip_fail(ctx, e, "render wave");
+        // film updates in wave order, so the film is summed in sample order
+        if (w > 0 && nstreams > 1 && (e = hipStreamWaitEvent(st, sc->ev_film, 0)) != hipSuccess)
+            return hip_fail(ctx, e, "hipStreamWaitEvent");
+        if ((e = pt_wave_film(L, st)) != hipSuccess) return hip_fail(ctx, e, "render film");
+        if (nstreams > 1 && (e = hipEventRecord(sc->ev_film, st)) != hipSuccess)
+            return hip_fail(ctx, e, "hipEventRecord");
+        if ((rc = sc->ring.done(ctx, st))) return rc;
+    }
+    if (nstreams > 1) {  // join: the caller's stream waits for both internal streams
+        for (int k = 0; k < nstreams; k++)
+            if ((e = hipEventRecord(sc->ev_join[k], sc->rs[k])) != hipSuccess ||
+                (e = hipStreamWaitEvent(s, sc->ev_join[k], 0)) != hipSuccess)
+                return hip_fail(ctx, e, "stream join");
     }
     if (d_film_rgb && (e = pt_resolve_film(sc->film, npix, d_film_rgb, s)) != hipSuccess)
         return hip_fail(ctx, e, "resolve film");
@@ -247,7 +302,7 @@ int sortgpu_li(sortgpu_scene* sc, const sortgpu_ray* d_rays, const uint32_t* d_k
     }
     int rc = ctx_bind(ctx);
     if (rc) return rc;
-    if ((rc = sc->ws.ensure(ctx, n))) return rc;
+    if ((rc = sc->ws[0].ensure(ctx, n))) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     WaveDesc wd;
     std::memset(&wd, 0, sizeof(wd));
@@ -261,8 +316,10 @@ int sortgpu_li(sortgpu_scene* sc, const sortgpu_ray* d_rays, const uint32_t* d_k
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
     if ((rc = sc->ring.push(ctx, wd, s, nullptr, &d_wave))) return rc;
     PtLaunch L = make_launch(sc, d_wave, max_depth);
-    if ((e = pt_zero_f4(reinterpret_cast<float4*>(sc->ws.q.totals), 1, s)) != hipSuccess)
-        return hip_fail(ctx, e, "zero totals");
+    for (int k = 0; k < sortgpu_scene::kStreams; k++)
+        if (sc->ws[k].q.totals &&
+            (e = pt_zero_f4(reinterpret_cast<float4*>(sc->ws[k].q.totals), 1, s)) != hipSuccess)
+            return hip_fail(ctx, e, "zero totals");
     if ((e = pt_li_rays(L, reinterpret_cast<const float4*>(d_rays), d_keys, n, d_rgb, s)) != hipSuccess)
         return hip_fail(ctx, e, "li");
     if ((rc = sc->ring.done(ctx, s))) return rc;
@@ -279,9 +336,13 @@ int sortgpu_render_stats(sortgpu_scene* sc, uint64_t out[4]) {
     hipError_t e = hipEventSynchronize(sc->ev1);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventSynchronize");
     unsigned long long t[2] = {0, 0};
-    if (sc->ws.q.totals) {
-        e = hipMemcpy(t, sc->ws.q.totals, sizeof(t), hipMemcpyDeviceToHost);
+    for (int k = 0; k < sortgpu_scene::kStreams; k++) {
+        if (!sc->ws[k].q.totals) continue;
+        unsigned long long tk[2] = {0, 0};
+        e = hipMemcpy(tk, sc->ws[k].q.totals, sizeof(tk), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(totals)");
+        t[0] += tk[0];
+        t[1] += tk[1];
     }
     float ms = 0.0f;
     e = hipEventElapsedTime(&ms, sc->ev0, sc->ev1);

@@ -130,3 +130,18 @@ def test_eviction_and_cancellation_leave_results_unchanged(gpu_ctx, world):
         ref = _plain(world, r["k"], r["cam"], r["w"], r["h"], r["spp"], r["depth"], r["seed"])
         assert np.array_equal(srv.result(rid).view(np.uint32), ref.view(np.uint32)), r
     srv.close()
+
+
+def test_multi_wave_render_equals_served(gpu_ctx, world):
+    """A frame too big for one wave of the single-request render (1024^2 at 8 spp: four waves of two
+    samples, alternating over two streams) equals, bit for bit, the server's result for the same
+    request (one 8-sample wave)."""
+    cam = I.camera(**world[0][1]["camera"])
+    plain = _plain(world, 0, cam, 1024, 1024, 8, 5, 77)
+    srv, ids = _server(gpu_ctx, world)
+    rid = srv.submit(ids[0], cam, 1024, 1024, 8, max_depth=5, seed=77)
+    assert srv.wait(rid, 120.0)["state"] == "done"
+    assert np.array_equal(srv.result(rid).view(np.uint32), plain.view(np.uint32))
+    st = world[0][2].stats()
+    assert st["paths"] == 1024 * 1024 * 8 and st["extension_rays"] > st["paths"]
+    srv.close()
