@@ -18,6 +18,10 @@
 // The walk is SORT's Fbvh::GetIntersect (accel/fast_bvh.hpp:305-504): pop, prune when the
 // current closest t is below the entry distance, test the leaf's primitives in order, or test
 // the W child boxes and push the hit children farthest first (ties: lower lane pushed first).
+// Per wave the loop alternates a node phase (walking lanes step inner nodes together) and a leaf
+// phase (lanes holding a leaf test one primitive record each per step); a lane loads its leaf's
+// next record while it tests the current one. Any-hit pushes eligible children in lane order:
+// with no closest-hit pruning before the first hit, occlusion does not depend on visit order.
 #include <hip/hip_runtime.h>
 
 #include "traverse.h"
@@ -51,6 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
 
     RayK r;
     uint32_t li = 0xffffffffu;  // next primitive record of this lane's leaf
+    PrimRec cur;                // that record, loaded ahead
     uint32_t ray_id = 0xffffffffu;
     bool more = true;
     int sp = 0;
@@ -155,6 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                     if (ref & REF_LEAF) {
                         if (STATS) n_leaves++;
                         li = ref & ~REF_LEAF;
+                        cur = load_prim(prims, li);
                     } else {
                         if (STATS) n_nodes++;
                         float f[W];
@@ -172,11 +178,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                         for (int j = 0; j < W; j++) {
                             if (elig & (1u << j)) {
                                 int rank = 0;
+                                if (ANY && !STATS) {
+                                    rank = __popc(elig & ((1u << j) - 1u));
+                                } else {
 #pragma unroll
-                                for (int k = 0; k < W; k++) {
-                                    if (k == j) continue;
-                                    const bool before = (f[k] > f[j]) | ((f[k] == f[j]) & (k < j));
-                                    rank += ((elig >> k) & 1u) & (before ? 1u : 0u);
+                                    for (int k = 0; k < W; k++) {
+                                        if (k == j) continue;
+                                        const bool before = (f[k] > f[j]) | ((f[k] == f[j]) & (k < j));
+                                        rank += ((elig >> k) & 1u) & (before ? 1u : 0u);
+                                    }
                                 }
                                 push(sp + rank, ch[j], f[j]);
                             }
@@ -202,7 +212,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
             const uint32_t cnt = (uint32_t)__popcll(holding);
             if (cnt < leaf_min && cnt < leaf_batch && __ballot(li == 0xffffffffu && sp > 0) != 0) break;
             if (li != 0xffffffffu) {
-                const PrimRec pr = load_prim(prims, li);
+                const PrimRec pr = cur;
+                if (!(pr.flags & PR_LAST)) cur = load_prim(prims, li + 1);
                 float t, u, v;
                 bool hit = false;
                 if (pr.flags & PR_QUAD) {

@@ -217,10 +217,15 @@ def run_served(server: Server, trace, scene_ids, time_scale=1.0, on_submit=None)
     return ids, out, wall
 
 
-def run_one_at_a_time(pts, trace, time_scale=1.0):
-    """The plain path: each request rendered alone (``PathTracing.render`` on its resident scene,
-    result copied to the host) in arrival order, starting when it has arrived and the previous one
-    is done. Returns (latencies in seconds, wall seconds, ray totals)."""
+def run_one_at_a_time(pts, trace, time_scale=1.0, per_job=None):
+    """The plain path: each request rendered alone (``PathTracing.render``, result copied to the
+    host) in arrival order, starting when it has arrived and the previous one is done. With
+    `per_job` = (ctx, built scenes), each request first builds and uploads its scene and frees it
+    afterwards, as one SORT render job does (the scene is loaded and its accelerator built per
+    job); otherwise `pts` holds every scene resident. Returns (latencies in seconds, wall seconds,
+    ray totals)."""
+    from .accel import Obvh
+    from .integrator import PathTracing
     t0 = time.perf_counter()
     lat, ext, shadow = [], 0, 0
     for r in trace:
@@ -228,12 +233,22 @@ def run_one_at_a_time(pts, trace, time_scale=1.0):
         now = time.perf_counter() - t0
         if now < arrive:
             time.sleep(arrive - now)
-        pt = pts[r["scene"]]
+        if per_job:
+            ctx, built = per_job
+            sc = built[r["scene"]][1]
+            acc = Obvh(ctx).build(sc["pos"], sc["idx"], quads=sc["quads"])
+            pt = PathTracing(acc, sc["materials"], tri_material=sc["tri_material"], quad_radiance=sc["quad_radiance"],
+                             env_radiance=sc["env"])
+        else:
+            pt = pts[r["scene"]]
         pt.render(r["cam"], r["width"], r["height"], r["spp"], seed=r["seed"], max_depth=r["depth"]).cpu()
         lat.append(time.perf_counter() - t0 - arrive)
         st = pt.stats()
         ext += st["extension_rays"]
         shadow += st["shadow_rays"]
+        if per_job:
+            pt.close()
+            acc.release()
     return np.array(lat), time.perf_counter() - t0, (ext, shadow)
 
 
@@ -277,13 +292,15 @@ def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, infligh
     for i, pt in enumerate(pts):
         pt.render(random_camera(np.random.default_rng(i)), 32, 32, 1).cpu()
     for name, ts in (("poisson", 1.0), ("saturated", 0.0)):
-        lat, wall, (ext, shadow) = run_one_at_a_time(pts, trace, time_scale=ts)
-        res[name]["one_at_a_time"] = {"p50_ms": round(_pct(lat, 50) * 1e3, 1), "p99_ms": round(_pct(lat, 99) * 1e3, 1),
-                                      "wall_s": round(wall, 3), "mpaths_s": round(paths / wall / 1e6, 2),
-                                      "mrays_s": round((ext + shadow) / wall / 1e6, 2)}
-        s, o = res[name]["served"], res[name]["one_at_a_time"]
-        res[name]["throughput_ratio"] = round(o["wall_s"] / s["wall_s"], 3)
-        log(f"[served] {name}: one-at-a-time {o}")
+        for key, job in (("one_at_a_time", None), ("one_at_a_time_per_job", (ctx, built))):
+            lat, wall, (ext, shadow) = run_one_at_a_time(pts, trace, time_scale=ts, per_job=job)
+            res[name][key] = {"p50_ms": round(_pct(lat, 50) * 1e3, 1), "p99_ms": round(_pct(lat, 99) * 1e3, 1),
+                              "wall_s": round(wall, 3), "mpaths_s": round(paths / wall / 1e6, 2),
+                              "mrays_s": round((ext + shadow) / wall / 1e6, 2)}
+            log(f"[served] {name}: {key.replace('_', '-')} {res[name][key]}")
+        s = res[name]["served"]
+        res[name]["throughput_ratio"] = round(res[name]["one_at_a_time"]["wall_s"] / s["wall_s"], 3)
+        res[name]["throughput_ratio_per_job"] = round(res[name]["one_at_a_time_per_job"]["wall_s"] / s["wall_s"], 3)
     for pt in pts:
         pt.close()
     res["server"] = {"wave_paths": wave_paths or (1 << 23), "inflight_waves": inflight_waves or 3, "graphs": graphs}
