@@ -9,11 +9,12 @@
 // scene's tree view (root, box, record pointers into the concatenated pool) when it takes the
 // ray, so rays of any number of scenes share one launch and one work queue.
 //
-// Traversal stack: entries (child reference, entry distance). The top LDS entries of every
-// lane live in LDS ([entry][thread] so a wave's accesses to one entry hit 64 consecutive
-// dwords); deeper entries spill to a private per-lane array in scratch. The host picks the
-// spill depth from the tree's exact worst case (W-1)*(depth-1)+1, so no push can go past it;
-// the kernel still bounds-checks every push and raises an error flag instead of writing.
+// Traversal stack: 8-byte entries (child reference, entry distance), so a pop is one LDS read.
+// The top LDS entries of every lane live in LDS ([entry][thread], so a wave's accesses to one
+// entry hit 64 consecutive entries); deeper entries spill to a private per-lane array in
+// scratch. The host picks the spill depth from the tree's exact worst case (W-1)*(depth-1)+1, so
+// no push can go past it; the kernel still bounds-checks every push and raises an error flag
+// instead of writing.
 //
 // The walk is SORT's Fbvh::GetIntersect (accel/fast_bvh.hpp:305-504): pop, prune when the
 // current closest t is below the entry distance, test the leaf's primitives in order, or test
@@ -22,6 +23,9 @@
 // phase (lanes holding a leaf test one primitive record each per step); a lane loads its leaf's
 // next record while it tests the current one. Any-hit pushes eligible children in lane order:
 // with no closest-hit pruning before the first hit, occlusion does not depend on visit order.
+// Slab tests of rays with finite components against trees with bounded boxes use IEEE
+// min/max (equal to the x86 selects when no operand is NaN, which such rays cannot produce);
+// other rays take the select form (traverse.h, ieee_slabs_ok).
 #include <hip/hip_runtime.h>
 
 #include "traverse.h"
@@ -35,13 +39,52 @@ constexpr int kBlock = 256;
 constexpr int kDefaultLds = 16;
 constexpr uint32_t kDefaultLeafBatch = 32;
 
+// One primitive record of a lane's leaf: `pr` is tested while the leaf's next record (if any) is
+// loaded into `nxt`. Closest hit keeps the first record with the smallest t in (tmin, tmax]
+// (strict t < bt for triangles; a quad at the current t replaces it); any hit ends the ray.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ void leaf_step(const RayK& r, const PrimRec* __restrict__ prims,
+                                          const QuadRec* __restrict__ quads, const PrimRec& pr, PrimRec& nxt,
+                                          uint32_t& li, int& sp, float& bt, uint32_t& bp, float& bu, float& bv,
+                                          bool& occluded, uint32_t& n_tris, uint32_t& n_quads) {
+    if (li == 0xffffffffu) return;
+    if (!(pr.flags & PR_LAST)) nxt = load_prim(prims, li + 1);
+    float t, u, v;
+    bool hit = false;
+    if (pr.flags & PR_QUAD) {
+        if (STATS) n_quads++;
+        if (quad_test(r, quads[pr.aux], bt, t)) {
+            bt = t;
+            bp = pr.prim;
+            bu = bv = 0.f;
+            hit = true;
+        }
+    } else {
+        if (STATS) n_tris++;
+        if (tri_test(r, pr, t, u, v) && t < bt) {
+            bt = t;
+            bp = pr.prim;
+            bu = u;
+            bv = v;
+            hit = true;
+        }
+    }
+    if (ANY && hit) {
+        occluded = true;
+        li = 0xffffffffu;
+        sp = 0;
+    } else {
+        li = (pr.flags & PR_LAST) ? 0xffffffffu : li + 1;
+    }
+}
+
 template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS>
-__global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
-    __shared__ uint32_t s_ref[LDS * kBlock];
-    __shared__ float s_tm[LDS * kBlock];
+// At least 3 waves per SIMD (at most 168 VGPRs): the occupancy every W = 8 form is tuned for.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_trace(TraceArgs a) {
+    // stack entry: (child reference, entry distance bits)
+    __shared__ uint2 s_st[LDS * kBlock];
     constexpr int CAP = LDS + SPILL;
-    uint32_t sp_ref[SPILL];
-    float sp_tm[SPILL];
+    uint2 sp_st[SPILL];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
@@ -55,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
 
     RayK r;
     uint32_t li = 0xffffffffu;  // next primitive record of this lane's leaf
-    PrimRec cur;                // that record, loaded ahead
+    PrimRec cur, alt;           // that record, loaded ahead; the leaf phase alternates the two
     uint32_t ray_id = 0xffffffffu;
     bool more = true;
     int sp = 0;
@@ -65,15 +108,26 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
     bool overflow = false;
 
     auto push = [&](int e, uint32_t ref, float t) {
+        const uint2 v = make_uint2(ref, __float_as_uint(t));
         if (e < LDS) {
-            s_ref[e * kBlock + tid] = ref;
-            s_tm[e * kBlock + tid] = t;
+            s_st[e * kBlock + tid] = v;
         } else if (e < CAP) {
-            sp_ref[e - LDS] = ref;
-            sp_tm[e - LDS] = t;
+            sp_st[e - LDS] = v;
         } else {
             overflow = true;
         }
+    };
+
+    bool occluded = false;
+    // Whether the leaf phase takes another step. It may stop while leaves are unfinished only
+    // when fewer than `leaf_min` lanes hold one and the next node phase can take a step (some lane
+    // can walk and fewer than `leaf_batch` lanes hold a leaf), so every round advances some lane
+    // whatever the knobs.
+    auto leaf_go = [&]() -> bool {
+        const uint64_t holding = __ballot(li != 0xffffffffu);
+        if (holding == 0) return false;
+        const uint32_t cnt = (uint32_t)__popcll(holding);
+        return !(cnt < leaf_min && cnt < leaf_batch && __ballot(li == 0xffffffffu && sp > 0) != 0);
     };
 
     while (true) {
@@ -110,6 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                             root = tv.root;
                             valid = tv.valid;
                             if (valid) rmin = root_slab(r, tv.bbox);
+                            r.ieee = ieee_slabs_ok(r, tv.bbox);
                         } else {
                             root = 0;
                             valid = 0;
@@ -119,6 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                         root = a.tree.root;
                         valid = a.tree.valid;
                         if (valid) rmin = root_slab(r, a.tree.bbox);
+                        r.ieee = ieee_slabs_ok(r, a.tree.bbox);
                     }
                     if (rmin >= 0.0f) {
                         push(0, root, rmin);
@@ -146,16 +202,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
             if (__ballot(walk) == 0 || (uint32_t)__popcll(__ballot(li != 0xffffffffu)) >= leaf_batch) break;
             if (walk) {
                 --sp;
-                uint32_t ref;
-                float tm;
-                if (sp < LDS) {
-                    ref = s_ref[sp * kBlock + tid];
-                    tm = s_tm[sp * kBlock + tid];
-                } else {
-                    const int e = sp - LDS < SPILL ? sp - LDS : 0;
-                    ref = sp_ref[e];
-                    tm = sp_tm[e];
-                }
+                // the LDS read is unconditional (clamped index) so that it stays an LDS access
+                uint2 e = s_st[(sp < LDS ? sp : LDS - 1) * kBlock + tid];
+                if (sp >= LDS) e = sp_st[sp - LDS < SPILL ? sp - LDS : 0];
+                const uint32_t ref = e.x;
+                const float tm = __uint_as_float(e.y);
                 if (!(bt < tm)) {
                     if (ref & REF_LEAF) {
                         if (STATS) n_leaves++;
@@ -174,21 +225,28 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
                             for (int j = 0; j < W; j++)
                                 if (!(f[j] > -1.0f)) elig &= ~(1u << j);
                         }
+                        // Push order: farther first, ties lower lane first. An eligible child hit its
+                        // box (fmn <= fmx), so its distance is not NaN and the order is total: one
+                        // comparison per pair decides which of the two goes first.
+                        uint32_t rank[W];
+#pragma unroll
+                        for (int j = 0; j < W; j++) rank[j] = 0;
+                        if (!(ANY && !STATS)) {
+#pragma unroll
+                            for (int j = 1; j < W; j++) {
+#pragma unroll
+                                for (int k = 0; k < j; k++) {
+                                    const bool kfirst = f[k] >= f[j];
+                                    rank[j] += kfirst ? ((elig >> k) & 1u) : 0u;
+                                    rank[k] += kfirst ? 0u : ((elig >> j) & 1u);
+                                }
+                            }
+                        }
 #pragma unroll
                         for (int j = 0; j < W; j++) {
                             if (elig & (1u << j)) {
-                                int rank = 0;
-                                if (ANY && !STATS) {
-                                    rank = __popc(elig & ((1u << j) - 1u));
-                                } else {
-#pragma unroll
-                                    for (int k = 0; k < W; k++) {
-                                        if (k == j) continue;
-                                        const bool before = (f[k] > f[j]) | ((f[k] == f[j]) & (k < j));
-                                        rank += ((elig >> k) & 1u) & (before ? 1u : 0u);
-                                    }
-                                }
-                                push(sp + rank, ch[j], f[j]);
+                                const uint32_t rk = (ANY && !STATS) ? (uint32_t)__popc(elig & ((1u << j) - 1u)) : rank[j];
+                                push(sp + (int)rk, ch[j], f[j]);
                             }
                         }
                         sp += __popc(elig);
@@ -201,47 +259,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
         }
         // Leaf phase: the lanes holding a leaf test its primitive records together, one record
         // per step, in record order (a lane keeps its place in an unfinished leaf for the next
-        // round and walks no node before finishing it). It may stop while leaves are unfinished
-        // only when fewer than `leaf_min` lanes hold one and the next node phase can take a step
-        // (some lane can walk and fewer than `leaf_batch` lanes hold a leaf), so every round
-        // advances some lane whatever the knobs.
-        bool occluded = false;
+        // round and walks no node before finishing it). Two steps per iteration, so that the
+        // record under test and the one loading swap roles without register copies.
         while (true) {
-            const uint64_t holding = __ballot(li != 0xffffffffu);
-            if (holding == 0) break;
-            const uint32_t cnt = (uint32_t)__popcll(holding);
-            if (cnt < leaf_min && cnt < leaf_batch && __ballot(li == 0xffffffffu && sp > 0) != 0) break;
-            if (li != 0xffffffffu) {
-                const PrimRec pr = cur;
-                if (!(pr.flags & PR_LAST)) cur = load_prim(prims, li + 1);
-                float t, u, v;
-                bool hit = false;
-                if (pr.flags & PR_QUAD) {
-                    if (STATS) n_quads++;
-                    if (quad_test(r, quads[pr.aux], bt, t)) {
-                        bt = t;
-                        bp = pr.prim;
-                        bu = bv = 0.f;
-                        hit = true;
-                    }
-                } else {
-                    if (STATS) n_tris++;
-                    if (tri_test(r, pr, t, u, v) && t < bt) {
-                        bt = t;
-                        bp = pr.prim;
-                        bu = u;
-                        bv = v;
-                        hit = true;
-                    }
-                }
-                if (ANY && hit) {
-                    occluded = true;
-                    li = 0xffffffffu;
-                    sp = 0;
-                } else {
-                    li = (pr.flags & PR_LAST) ? 0xffffffffu : li + 1;
-                }
+            if (!leaf_go()) break;
+            leaf_step<ANY, STATS>(r, prims, quads, cur, alt, li, sp, bt, bp, bu, bv, occluded, n_tris, n_quads);
+            if (!leaf_go()) {
+                cur = alt;
+                break;
             }
+            leaf_step<ANY, STATS>(r, prims, quads, alt, cur, li, sp, bt, bp, bu, bv, occluded, n_tris, n_quads);
         }
         if (occluded) sp = 0;
         if (ray_id != 0xffffffffu && sp == 0 && li == 0xffffffffu) {
@@ -249,6 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(TraceArgs a) {
             else reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);
             ray_id = 0xffffffffu;
         }
+        occluded = false;
     }
 
     if (overflow) atomicOr(a.error, 1u);

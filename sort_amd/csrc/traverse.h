@@ -16,7 +16,8 @@
 //   shadow early exit                   accel/fast_bvh.hpp:362-373
 // No FMA: this header is compiled under `#pragma clang fp contract(off)` and -ffp-contract=off.
 // Division is IEEE (HIP's default correctly rounded fp32 divide). min/max are written as the
-// x86 selects (_mm_min_ps(a,b) = a < b ? a : b).
+// x86 selects (_mm_min_ps(a,b) = a < b ? a : b), except in the slab test of rays for which no
+// slab distance can be NaN: there IEEE min/max (v_min/v_max/v_min3/v_max3) give the same values.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <float.h>
@@ -45,6 +46,7 @@ struct RayK {
     float okx, oky, okz;       // origin permuted into the shear frame
     float rcx, rcy, rcz;       // 1 / clamped dir
     float odx, ody, odz;       // -o / clamped dir
+    bool ieee;                 // slab tests may use IEEE min/max (see ieee_slabs_ok)
 };
 
 __device__ __forceinline__ void ray_setup(RayK& r) {
@@ -210,26 +212,63 @@ __device__ __forceinline__ uint32_t node_test(const RayK& r, const NodeRec<W>* _
         ch[4 * v + 3] = q.w;
     }
     uint32_t mask = 0;
+    if (r.ieee) {
 #pragma unroll
-    for (int j = 0; j < W; j++) {
-        float fmn = r.tmin, fmx = r.tmax;
-        float t1 = r.odx + r.rcx * pl[1][j];
-        float t2 = r.odx + r.rcx * pl[0][j];
-        fmn = xmax(fmn, xmin(t1, t2));
-        fmx = xmin(fmx, xmax(t1, t2));
-        t1 = r.ody + r.rcy * pl[3][j];
-        t2 = r.ody + r.rcy * pl[2][j];
-        fmn = xmax(fmn, xmin(t1, t2));
-        fmx = xmin(fmx, xmax(t1, t2));
-        t1 = r.odz + r.rcz * pl[5][j];
-        t2 = r.odz + r.rcz * pl[4][j];
-        fmn = xmax(fmn, xmin(t1, t2));
-        fmx = xmin(fmx, xmax(t1, t2));
-        const bool hit = (ch[j] != REF_EMPTY) & (fmn <= fmx);
-        f[j] = hit ? fmn : -1.0f;
-        mask |= hit ? (1u << j) : 0u;
+        for (int j = 0; j < W; j++) {
+            const float x1 = r.odx + r.rcx * pl[1][j], x2 = r.odx + r.rcx * pl[0][j];
+            const float y1 = r.ody + r.rcy * pl[3][j], y2 = r.ody + r.rcy * pl[2][j];
+            const float z1 = r.odz + r.rcz * pl[5][j], z2 = r.odz + r.rcz * pl[4][j];
+            const float fmn = fmaxf(fmaxf(fmaxf(r.tmin, fminf(x1, x2)), fminf(y1, y2)), fminf(z1, z2));
+            const float fmx = fminf(fminf(fminf(r.tmax, fmaxf(x1, x2)), fmaxf(y1, y2)), fmaxf(z1, z2));
+            const bool hit = (ch[j] != REF_EMPTY) & (fmn <= fmx);
+            f[j] = hit ? fmn : -1.0f;
+            mask |= hit ? (1u << j) : 0u;
+        }
+    } else {
+        // Opaque copies: the compiler must not share this rare path's slab distances with the
+        // path above (hoisting them ahead of the branch keeps all 6*W of them live at once).
+        float odx = r.odx, ody = r.ody, odz = r.odz, rcx = r.rcx, rcy = r.rcy, rcz = r.rcz;
+        asm volatile("" : "+v"(odx), "+v"(ody), "+v"(odz), "+v"(rcx), "+v"(rcy), "+v"(rcz));
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            float fmn = r.tmin, fmx = r.tmax;
+            float t1 = odx + rcx * pl[1][j];
+            float t2 = odx + rcx * pl[0][j];
+            fmn = xmax(fmn, xmin(t1, t2));
+            fmx = xmin(fmx, xmax(t1, t2));
+            t1 = ody + rcy * pl[3][j];
+            t2 = ody + rcy * pl[2][j];
+            fmn = xmax(fmn, xmin(t1, t2));
+            fmx = xmin(fmx, xmax(t1, t2));
+            t1 = odz + rcz * pl[5][j];
+            t2 = odz + rcz * pl[4][j];
+            fmn = xmax(fmn, xmin(t1, t2));
+            fmx = xmin(fmx, xmax(t1, t2));
+            const bool hit = (ch[j] != REF_EMPTY) & (fmn <= fmx);
+            f[j] = hit ? fmn : -1.0f;
+            mask |= hit ? (1u << j) : 0u;
+        }
     }
     return mask;
+}
+
+// Whether IEEE min/max may stand in for the x86 selects in this ray's slab tests against a tree
+// with root box `bb`. The two differ only when an operand is NaN (and in the sign of a zero
+// result, which only ever enters comparisons). With a finite origin and direction, the clamped
+// direction d' has |d'| >= 1e-5, so 1/d' is finite (|1/d'| <= 1e5) and -o/d' is finite or
+// infinite but not NaN; with every box coordinate p within kIeeeBoxBound, (1/d') * p is finite,
+// so -o/d' + (1/d') * p is never inf - inf and no slab distance is NaN; a window with no NaN adds
+// none. Empty child slots may produce anything: the REF_EMPTY test discards their result.
+// Any bound below FLT_MAX / 1e5 is sound; 1e12 is far beyond every float32 scene (at 5e12 the
+// triangle test's t numerator already overflows) and lets a tree with real hits sit above it.
+constexpr float kIeeeBoxBound = 1e12f;
+__device__ __forceinline__ bool ieee_slabs_ok(const RayK& r, const float* bb) {
+    bool ok = __builtin_isfinite(r.ox) & __builtin_isfinite(r.oy) & __builtin_isfinite(r.oz) &
+              __builtin_isfinite(r.dx) & __builtin_isfinite(r.dy) & __builtin_isfinite(r.dz) &
+              !__builtin_isnan(r.tmin) & !__builtin_isnan(r.tmax);
+#pragma unroll
+    for (int k = 0; k < 6; k++) ok &= fabsf(bb[k]) <= kIeeeBoxBound;
+    return ok;
 }
 
 }  // namespace sortgpu

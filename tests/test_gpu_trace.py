@@ -25,7 +25,7 @@ def _pair(ctx, pos, idx, quads=None, width=8, **kw):
     return acc, ob
 
 
-def _check(acc, ob, rays, label=""):
+def _check(acc, ob, rays, label="", window_property=True):
     print(f"[parity] {label} width={acc.width} rays={len(rays)}", flush=True)
     d_rays = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
     hits = acc.get_intersect(d_rays).cpu().numpy()
@@ -36,8 +36,9 @@ def _check(acc, ob, rays, label=""):
     assert bad.size == 0, f"{label}: {bad.size} closest mismatches, first {bad[:5]}: gpu {hits[bad[:3]]} ref {ref[bad[:3]]}"
     bad = np.nonzero(occ != ref_occ)[0]
     assert bad.size == 0, f"{label}: {bad.size} any-hit mismatches, first {bad[:5]}"
-    # size-independent property: occluded <=> a closest hit exists in the same window
-    assert np.array_equal(occ.astype(bool), hits[:, 1].view(np.uint32) != MISS)
+    if window_property:
+        # size-independent property: occluded <=> a closest hit exists in the same window
+        assert np.array_equal(occ.astype(bool), hits[:, 1].view(np.uint32) != MISS)
     return hits, occ
 
 
@@ -135,6 +136,41 @@ def test_axis_parallel_and_windowed_rays(gpu_ctx):
     tmin = g.uniform(0, 0.3, n).astype(np.float32)
     tmax = (tmin + g.uniform(0, 1.0, n)).astype(np.float32)
     _check(acc, ob, scenes.make_rays(o, d, tmin, tmax), "axis parallel windowed")
+
+
+@pytest.mark.parametrize("width", [8, 4])
+def test_select_form_slab_tests(gpu_ctx, width):
+    """Rays with infinite or NaN components, and every ray of a tree whose box reaches past 1e12,
+    take the x86-select form of the slab test (the IEEE min/max form is only used where no slab
+    distance can be NaN); both forms must give the oracle's bits. A ray's results must not depend
+    on which form its wave neighbours use, so ordinary and non-finite rays are interleaved."""
+    pos, idx = scenes.soup(1 << 13, seed=31)
+    acc, ob = _pair(gpu_ctx, pos, idx, width=width)
+    rays = scenes.random_rays(16384, 32, lo=-0.2, hi=1.2)
+    inf, nan = np.float32(np.inf), np.float32(np.nan)
+    r = rays.copy()
+    n = len(r)
+    k = np.arange(n)
+    r[k % 8 == 1, 0] = inf                       # infinite origin
+    r[k % 8 == 2, 5] = -inf                      # infinite direction
+    r[k % 8 == 3, 1] = nan                       # NaN origin
+    r[k % 8 == 4, 6] = nan                       # NaN direction
+    sel = k % 8 == 5                             # infinite origin and direction on one axis
+    r[sel, 2] = -inf
+    r[sel, 6] = inf
+    r[k % 8 == 6, 7] = nan                       # NaN tmax
+    r[k % 16 == 7, 3] = nan                      # NaN tmin
+    _check(acc, ob, r, "non-finite rays", window_property=False)
+    # the same soup and rays scaled by 2e12: a normal tree (SAH areas and the triangle test's
+    # t numerator stay finite) whose box is past the IEEE bound, so every ray takes the select form
+    s = np.float32(2e12)
+    acc2, ob2 = _pair(gpu_ctx, (pos * s).astype(np.float32), idx, width=width)
+    assert acc2.info()["inner_nodes"] > 100
+    far = rays.copy()
+    far[:, :3] *= s
+    far[:, 7] = np.float32(3e38)
+    hits, _ = _check(acc2, ob2, far, "scaled by 2e12")
+    assert np.mean(hits[:, 1].view(np.uint32) != MISS) > 0.01
 
 
 @pytest.mark.parametrize("width", [8, 4])
