@@ -129,10 +129,29 @@ _SIGS = {
 }
 
 
-def header_symbols() -> list[str]:
-    """Every function the public header declares."""
-    text = open(HEADER).read()
+def header_symbols(header: str = HEADER) -> list[str]:
+    """Every function a header declares (default: the public header)."""
+    text = open(header).read()
     return sorted(set(re.findall(r"\b(sortgpu_[a-z0-9_]+)\s*\(", text)))
+
+
+PROBE_LIB_PATH = os.path.join(PKG, "libsortgpu_probe.so")
+PROBE_HEADER = os.path.join(os.path.dirname(PKG), "include", "sortgpu_probe.h")
+_probe = None
+
+
+def probe_lib() -> ctypes.CDLL:
+    """The vector-memory probe library (measurement tool, not the drop-in boundary)."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_LIB_PATH):
+            raise ImportError(f"{PROBE_LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        lib()  # same runtime-order rule as the library
+        l = ctypes.CDLL(PROBE_LIB_PATH)
+        l.sortgpu_probe_chunks.restype = _I
+        l.sortgpu_probe_chunks.argtypes = [_P, ctypes.c_uint32, _I, _I, _I, _P, _P]
+        _probe = l
+    return _probe
 
 
 def lib() -> ctypes.CDLL:

@@ -39,13 +39,16 @@ DEVICE = [
 HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp", "pool.cpp", "serve_engine.cpp", "serve_api.cpp"]
 # compiled as HIP: kernels, and host code that shares device record types with them
 DEVICE_SOURCES = ["trace.hip", "pt.hip", "pt_host.cpp", "pt_api.cpp", "serve_device.cpp"]
+# the vector-memory probe bench.py measures the traversal's ceiling with (its own library)
+PROBE_SOURCES = ["probe.hip"]
+PROBE_LIB = os.path.join(PKG, "libsortgpu_probe.so")
 
 
 def _compile(src: str) -> str:
     os.makedirs(OBJ, exist_ok=True)
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON
-    if src in DEVICE_SOURCES:
+    if src in DEVICE_SOURCES or src in PROBE_SOURCES:
         cmd += DEVICE + ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -54,19 +57,25 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every source and links ``sort_amd/libsortgpu.so``; returns its path."""
-    srcs = HOST_SOURCES + DEVICE_SOURCES
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+def _link(objs: list[str], out: str) -> None:
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every source and links ``sort_amd/libsortgpu.so`` (and the measurement probe
+    ``sort_amd/libsortgpu_probe.so``); returns the library's path."""
+    srcs = HOST_SOURCES + DEVICE_SOURCES + PROBE_SOURCES
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = dict(zip(srcs, ex.map(_compile, srcs)))
+    _link([objs[s] for s in HOST_SOURCES + DEVICE_SOURCES], LIB)
+    _link([objs[s] for s in PROBE_SOURCES], PROBE_LIB)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {PROBE_LIB}")
     return LIB
 
 

@@ -15,6 +15,12 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 13
     for s in syms:
         assert hasattr(l, s), s
+    # the measurement probe ships as its own library with its own header
+    from sort_amd._lib import PROBE_HEADER, probe_lib
+    p = probe_lib()
+    for s in header_symbols(PROBE_HEADER):
+        assert hasattr(p, s), s
+    assert p.sortgpu_probe_chunks(None, 1, 3, 1, 1, None, None) != 0  # refused before any launch
 
 
 def test_host_only_context_and_errors(host_ctx):
