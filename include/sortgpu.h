@@ -129,15 +129,21 @@ int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint
  * (default 16, 1..64); lds_entries = traversal-stack entries per lane kept in LDS (8 or 16,
  * default 16); blocks_per_cu = cap on resident workgroups per CU (default: occupancy limit);
  * leaf_batch = lanes of a wave holding a leaf that end a node-walking phase (default 32, 1..64);
- * leaf_min = a leaf-testing phase ends when fewer lanes hold a leaf (default 1, 1..64). The knobs
- * change speed only: every hit is the same for every setting. */
+ * leaf_min = a leaf-testing phase ends when fewer lanes hold a leaf (default 1, 1..64);
+ * reorder = traversal order of sortgpu_trace_closest / sortgpu_trace_any batches: 0 (default)
+ * sorts batches of at least 262,144 rays by origin cell (64^3 grid over the scene box) before the
+ * launch (stream-ordered device memory of 16 bytes per ray plus sort workspace, from a pool the
+ * context owns; a batch sharing one origin, such as camera rays, keeps its order and pays only the
+ * sort), 1 = never, 2 = always. The knobs change speed only: every hit is the same for every
+ * setting. */
 typedef struct sortgpu_tuning {
     uint32_t refill;
     uint32_t lds_entries;
     uint32_t blocks_per_cu;
     uint32_t leaf_batch;
     uint32_t leaf_min;
-    uint32_t reserved[3];
+    uint32_t reorder;
+    uint32_t reserved[2];
 } sortgpu_tuning;
 
 /* Applies to every later query, render and pool trace of the context. */

@@ -144,23 +144,29 @@ class Accelerator:
         arr = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 8))
         return torch.from_numpy(arr).to(f"cuda:{self.ctx.device}"), True
 
-    def get_intersect(self, rays, stream=None):
-        """Closest hits, (n,4) float32 records [t, prim bits, u, v]."""
+    def get_intersect(self, rays, stream=None, out=None):
+        """Closest hits, (n,4) float32 records [t, prim bits, u, v] (written into `out`, a
+        contiguous (n,4) float32 device tensor, when given)."""
         import torch
         d_rays, host = self._device_rays(rays)
         n = d_rays.shape[0]
-        hits = torch.empty((n, 4), dtype=torch.float32, device=d_rays.device)
+        if out is not None:
+            assert out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and out.numel() == 4 * n
+        hits = out if out is not None else torch.empty((n, 4), dtype=torch.float32, device=d_rays.device)
         s = stream if stream is not None else torch.cuda.current_stream(d_rays.device).cuda_stream
         check(lib().sortgpu_trace_closest(self.handle, d_rays.data_ptr(), n, hits.data_ptr(), s),
               self.ctx.handle)
         return hits.cpu().numpy() if host else hits
 
-    def is_occluded(self, rays, stream=None):
-        """Any-hit: uint8 per ray, 1 when some primitive lies in (tmin, tmax]."""
+    def is_occluded(self, rays, stream=None, out=None):
+        """Any-hit: uint8 per ray, 1 when some primitive lies in (tmin, tmax] (written into `out`,
+        a contiguous (n,) uint8 device tensor, when given)."""
         import torch
         d_rays, host = self._device_rays(rays)
         n = d_rays.shape[0]
-        occ = torch.empty((n,), dtype=torch.uint8, device=d_rays.device)
+        if out is not None:
+            assert out.is_cuda and out.dtype == torch.uint8 and out.is_contiguous() and out.numel() == n
+        occ = out if out is not None else torch.empty((n,), dtype=torch.uint8, device=d_rays.device)
         s = stream if stream is not None else torch.cuda.current_stream(d_rays.device).cuda_stream
         check(lib().sortgpu_trace_any(self.handle, d_rays.data_ptr(), n, occ.data_ptr(), s), self.ctx.handle)
         return occ.cpu().numpy() if host else occ

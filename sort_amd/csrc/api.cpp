@@ -122,9 +122,10 @@ int sortgpu_ctx_set_tuning(sortgpu_ctx* ctx, const sortgpu_tuning* t) {
         return SORTGPU_ERR_ARG;
     }
     if (t->refill > 64 || (t->lds_entries != 0 && t->lds_entries != 8 && t->lds_entries != 16) || t->leaf_batch > 64 ||
-        t->leaf_min > 64) {
+        t->leaf_min > 64 || t->reorder > 2) {
         set_error(ctx,
-                  "sortgpu_ctx_set_tuning: refill, leaf_batch and leaf_min must be <= 64 and lds_entries 0, 8 or 16");
+                  "sortgpu_ctx_set_tuning: refill, leaf_batch and leaf_min must be <= 64, lds_entries 0, 8 or 16 and "
+                  "reorder 0, 1 or 2");
         return SORTGPU_ERR_ARG;
     }
     ctx->tuning = *t;
@@ -259,6 +260,32 @@ int64_t sortgpu_accel_tree_dump(const sortgpu_accel* acc, uint32_t* out, uint64_
     return (int64_t)d.size();
 }
 
+// Batches at least this large are traced in coherence order by default (smaller ones do not pay
+// for the sort).
+constexpr uint32_t kReorderMinRays = 1u << 18;
+
+// The context's pool for traversal-order buffers: created on first use, on the context's device,
+// keeping freed memory for the next batch (release threshold at its maximum). Null if the device
+// has no stream-ordered allocator.
+static hipMemPool_t order_pool(sortgpu_ctx* ctx) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (!ctx->order_pool) {
+        hipMemPoolProps p{};
+        p.allocType = hipMemAllocationTypePinned;
+        p.location.type = hipMemLocationTypeDevice;
+        p.location.id = ctx->device;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &p) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        ctx->order_pool = pool;
+    }
+    return ctx->order_pool;
+}
+
 static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, void* d_out, bool any,
                         bool stats, uint64_t* out4, void* stream) {
     if (!acc) {
@@ -297,7 +324,17 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(stats)");
         a.stats = acc->d_stats;
     }
+    // traversal order (results do not depend on it; the counter launch keeps index order)
+    RayOrder ro;
+    const uint32_t reorder = ctx->tuning.reorder;
+    if (!stats && reorder != 1 && (reorder == 2 || n >= kReorderMinRays) && n <= 0x7fffffffu) {
+        hipMemPool_t pool = order_pool(ctx);
+        // an order that cannot be had (no pool, no memory) only costs speed: trace in index order
+        if (pool && order_rays(pool, d_rays, n, acc->view.bbox, s, ro) == hipSuccess) a.order = ro.order;
+        else (void)hipGetLastError();
+    }
     hipError_t e = launch_trace(a, any, stats, s, ctx->num_cus);
+    if (ro.order) release_order(ro, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
     if (stats) {
         unsigned long long h[4];

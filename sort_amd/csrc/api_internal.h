@@ -21,6 +21,12 @@ struct sortgpu_ctx {
     std::mutex mu;
     std::string err;
     std::atomic<int> refs{1};  // the creator's, plus one per object made from the context
+    // stream-ordered device memory for traversal orders (created on first use; released with the
+    // context once its last allocation is freed)
+    hipMemPool_t order_pool = nullptr;
+    ~sortgpu_ctx() {
+        if (order_pool) (void)hipMemPoolDestroy(order_pool);
+    }
 };
 
 namespace sortgpu {

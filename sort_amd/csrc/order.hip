@@ -1,0 +1,92 @@
+// Traversal order of a ray batch.
+//
+// A traversal lane reads its own node and primitive records, and what a launch costs is the
+// number of such reads that the compute units serve. Lanes of one wave that walk the same nodes
+// at the same time share those reads. Incoherent batches (bounce rays from random surface points)
+// arrive in no useful order, so before the launch the batch is sorted (stably) by its origin's
+// cell on a 64^3 grid over the tree's box, in Morton order (an 18-bit key). Direction cells in
+// the key were measured to cost camera batches 4% and gain bounce batches under 1%, so the key
+// has none: a batch sharing one origin keeps its order. The kernel takes rays in the sorted
+// order; each ray's result is written at its own index and does not depend on the order, so
+// results are bit-identical with or without it.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "trace_launch.h"
+
+namespace sortgpu {
+namespace {
+
+// bits of x (< 2^10) moved to every third position
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// cell index in [0, cells) of t in [0, 1]; NaN and out-of-range values clamp to an end cell
+__device__ __forceinline__ uint32_t cell(float t, float cells) {
+    const float c = fminf(fmaxf(t * cells, 0.0f), cells - 1.0f);  // fmaxf(NaN, 0) = 0
+    return (uint32_t)c;
+}
+
+__global__ __launch_bounds__(256) void k_ray_keys(const float4* __restrict__ rays, uint32_t n, float lx, float ly,
+                                                  float lz, float sx, float sy, float sz, uint32_t* __restrict__ keys,
+                                                  uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q0 = rays[2 * (size_t)i];
+    keys[i] = spread3(cell((q0.x - lx) * sx, 64.f)) | (spread3(cell((q0.y - ly) * sy, 64.f)) << 1) |
+              (spread3(cell((q0.z - lz) * sz, 64.f)) << 2);
+    vals[i] = i;
+}
+
+constexpr int kKeyBits = 18;
+
+}  // namespace
+
+void release_order(RayOrder& ro, hipStream_t s) {
+    for (void* p : {static_cast<void*>(ro.keys[0]), static_cast<void*>(ro.keys[1]), static_cast<void*>(ro.vals[0]),
+                    static_cast<void*>(ro.vals[1]), ro.temp})
+        if (p) (void)hipFreeAsync(p, s);
+    ro = RayOrder{};
+}
+
+hipError_t order_rays(hipMemPool_t pool, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
+                      RayOrder& ro) {
+    ro = RayOrder{};
+    const size_t b = (size_t)n * sizeof(uint32_t);
+    hipError_t e = hipSuccess;
+    for (uint32_t** p : {&ro.keys[0], &ro.keys[1], &ro.vals[0], &ro.vals[1]}) {
+        if ((e = hipMallocFromPoolAsync(reinterpret_cast<void**>(p), b, pool, s)) != hipSuccess) {
+            release_order(ro, s);
+            return e;
+        }
+    }
+    float ext[3];
+    for (int k = 0; k < 3; k++) {
+        const float w = bbox[3 + k] - bbox[k];
+        ext[k] = w > 0.0f ? 1.0f / w : 0.0f;
+    }
+    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, s, static_cast<const float4*>(rays), n, bbox[0],
+                       bbox[1], bbox[2], ext[0], ext[1], ext[2], ro.keys[0], ro.vals[0]);
+    if ((e = hipGetLastError()) != hipSuccess) {
+        release_order(ro, s);
+        return e;
+    }
+    hipcub::DoubleBuffer<uint32_t> kb(ro.keys[0], ro.keys[1]), vb(ro.vals[0], ro.vals[1]);
+    size_t tb = 0;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (int)n, 0, kKeyBits, s)) != hipSuccess ||
+        (e = hipMallocFromPoolAsync(&ro.temp, tb ? tb : 1, pool, s)) != hipSuccess ||
+        (e = hipcub::DeviceRadixSort::SortPairs(ro.temp, tb, kb, vb, (int)n, 0, kKeyBits, s)) != hipSuccess) {
+        release_order(ro, s);
+        return e;
+    }
+    ro.order = vb.Current();
+    return hipSuccess;
+}
+
+}  // namespace sortgpu

@@ -141,8 +141,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             if (lane == leader) base = atomicAdd(a.counter, cnt);
             base = __shfl(base, leader);
             if (idle && more) {
-                const uint32_t my = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
-                if (my < n_rays) {
+                const uint32_t k = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+                if (k < n_rays) {
+                    // the k-th ray of the traversal order (order has n entries, k < n)
+                    const uint32_t my = a.order ? a.order[k] : k;
                     ray_id = my;
                     const float4* rp = reinterpret_cast<const float4*>(a.rays) + 2 * (size_t)my;
                     const float4 q0 = rp[0], q1 = rp[1];

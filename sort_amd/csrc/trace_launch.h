@@ -28,10 +28,26 @@ struct TraceArgs {
     uint32_t blocks_per_cu;        // cap on resident workgroups per CU (default: occupancy limit)
     uint32_t leaf_batch;           // lanes holding a leaf that end a node phase (default 32)
     uint32_t leaf_min;             // a leaf phase ends below this many lanes holding a leaf (default 1)
+    // optional traversal order: the k-th ray taken from the work counter is rays[order[k]] (a
+    // permutation of 0..n-1, device), its result goes to out[order[k]]; null = index order.
+    const uint32_t* order = nullptr;
 };
 
 // Launches the closest (any = false) or any-hit traversal on `stream`.
 hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t stream, int num_cus);
+
+// Traversal order of a ray batch (order.hip): ray indices sorted by the origin's cell in the tree
+// box, so that a wave's lanes walk nearby nodes. Buffers come from `pool`, stream-ordered on `s`;
+// release_order frees them on `s` after the launch that reads `order`.
+struct RayOrder {
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    void* temp = nullptr;
+    const uint32_t* order = nullptr;
+};
+hipError_t order_rays(hipMemPool_t pool, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
+                      RayOrder& ro);
+void release_order(RayOrder& ro, hipStream_t s);
 // Deepest stack (entries) the kernels can hold.
 uint32_t trace_max_stack_supported();
 

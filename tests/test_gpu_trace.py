@@ -173,6 +173,48 @@ def test_select_form_slab_tests(gpu_ctx, width):
     assert np.mean(hits[:, 1].view(np.uint32) != MISS) > 0.01
 
 
+def test_traversal_order_changes_no_result(gpu_ctx):
+    """Large batches are traced in coherence order (sorted by origin and direction cell); the
+    order must not change a single bit of any result: never (1), always (2) and the default (0,
+    which sorts these 300K-ray batches) agree with each other and with the oracle, also for a
+    batch whose rays all share one sort key and for two streams tracing one accelerator at once
+    (each with its own stream-ordered sort buffers)."""
+    from sort_amd import batch
+    pos, idx = scenes.soup(1 << 16, seed=41)
+    acc, ob = _pair(gpu_ctx, pos, idx)
+    rays = scenes.random_rays(300_000, 42, lo=-0.2, hi=1.2)
+    one_key = np.repeat(rays[7:8], 1 << 18, axis=0)
+    d_rays, d_one = torch.from_numpy(rays).cuda(), torch.from_numpy(one_key).cuda()
+    res = {}
+    try:
+        for mode in (1, 2, 0):
+            batch.set_tuning(gpu_ctx, reorder=mode)
+            print(f"[order] mode {mode}", flush=True)
+            res[mode] = [acc.get_intersect(d_rays).cpu().numpy(), acc.is_occluded(d_rays).cpu().numpy(),
+                         acc.get_intersect(d_one).cpu().numpy(), acc.is_occluded(d_one).cpu().numpy()]
+        print("[order] two streams", flush=True)
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        h1 = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+        h2 = torch.empty((len(rays), 4), dtype=torch.float32, device="cuda")
+        back = torch.from_numpy(np.ascontiguousarray(rays[::-1])).cuda()
+        torch.cuda.synchronize()
+        for _ in range(3):
+            acc.get_intersect(d_rays, out=h1, stream=s1.cuda_stream)
+            acc.get_intersect(back, out=h2, stream=s2.cuda_stream)
+        torch.cuda.synchronize()
+        two = [h1.cpu().numpy(), h2.cpu().numpy()[::-1]]
+    finally:
+        batch.set_tuning(gpu_ctx)
+    for mode in (2, 0):
+        for a, b in zip(res[mode], res[1]):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), mode
+    for h in two:
+        assert np.array_equal(h.view(np.uint32), res[1][0].view(np.uint32))
+    ref = ob.closest(rays, threads=16)
+    assert np.array_equal(res[0][0].view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(res[0][1], ob.occluded(rays, threads=16))
+
+
 @pytest.mark.parametrize("width", [8, 4])
 def test_full_size_soup_parity(gpu_ctx, width):
     """The 1M-triangle workload at full size: 4M coherent, 4M incoherent and 4M shadow rays."""
