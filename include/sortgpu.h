@@ -115,6 +115,23 @@ int sortgpu_trace_closest(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint3
 int sortgpu_trace_any(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
                       uint8_t* d_occluded, void* stream);
 
+/* Transparent-shadow attenuation for n rays (Scene::GetAttenuation, core/scene.cpp:192-208,
+ * looping Accelerator::GetAttenuation, accel/accelerator.cpp:26-68, with the transparent-shadow
+ * walk of fast_bvh.hpp:352-405). d_transparency: 4 floats per primitive id (triangles, then quads):
+ * the material's transparency r, g, b and 1.0 when the material has transparency, else 0.0 (its
+ * transparency then counts as black). Each walk is a closest-hit search that stops, blocked, when
+ * a group of W consecutive leaf triangles, or a quad, makes the closest hit on a primitive without
+ * transparency (only each group's nearest hit is checked, as in SORT); a blocked ray gets black;
+ * a transparent closest hit multiplies the attenuation and the walk goes on from the hit point
+ * (o + t d) with (0.001, tmax - t), until a miss or a black attenuation. d_atten[4i..4i+3] = the
+ * attenuation r, g, b and the number of surfaces passed (as a float). A walk that passes 4,096
+ * surfaces stops there with its attenuation so far and sets the accelerator's error flag
+ * (reported by sortgpu_trace_counters). n_transparency = records in d_transparency; fewer than the
+ * accelerator's primitives (triangles + quads) is refused (SORTGPU_ERR_ARG). Stream-ordered. */
+int sortgpu_trace_attenuation(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
+                              const float* d_transparency, uint32_t n_transparency, float* d_atten,
+                              void* stream);
+
 /* Traversal work counters for n rays, synchronous: out[0..3] = inner nodes tested, leaves
  * visited, triangles tested, quads tested (the reference's OBVH statistics,
  * fast_bvh.hpp:74-97,407). mode 0 = closest, 1 = any-hit. Results are also written to

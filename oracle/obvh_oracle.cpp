@@ -611,8 +611,13 @@ template <int W> struct Bvh {
 
     // fast_bvh.hpp:305-504 (SIMD branch). shadow = SurfaceInteraction::query_shadow.
     // Returns: closest mode -> hit found ; shadow mode -> occluded.
+    // With `transp` (transparent shadows, the ENABLE_TRANSPARENT_SHADOW branches of
+    // fast_bvh.hpp:352-405): per primitive id [r, g, b, has transparency]; a blocked group or quad
+    // ends the walk only when its primitive has no transparency (then *opaque = true and the call
+    // returns true, the reference's primitive = nullptr code); otherwise the walk goes on as a
+    // closest-hit search and returns whether a hit was found.
     bool GetIntersect(Ray ray, Hit& hit, bool shadow, std::vector<std::pair<const Node<W>*, float>>& stack,
-                      Stats* st) const {
+                      Stats* st, const float* transp = nullptr, bool* opaque = nullptr) const {
         hit.t = FLT_MAX;
         hit.prim = 0xffffffffu;
         hit.u = hit.v = 0.f;
@@ -636,7 +641,12 @@ template <int W> struct Bvh {
                 for (const auto& g : node->tris) {
                     if (st) st->tris += g.count;
                     const bool blocked = IntersectTri(ray, sr, g, hit);
-                    if (shadow && blocked) return true;  // opaque blocker (fast_bvh.hpp:362-373)
+                    if (shadow && blocked) {  // fast_bvh.hpp:362-373
+                        if (!transp || transp[4 * (size_t)hit.prim + 3] == 0.0f) {
+                            if (opaque) *opaque = true;
+                            return true;
+                        }
+                    }
                 }
                 for (const uint32_t q : node->others) {
                     if (st) st->quads++;
@@ -645,7 +655,10 @@ template <int W> struct Bvh {
                         hit.t = tq;
                         hit.prim = sc.ntris + q;
                         hit.u = hit.v = 0.f;
-                        if (shadow) return true;
+                        if (shadow && (!transp || transp[4 * (size_t)hit.prim + 3] == 0.0f)) {  // :394-405
+                            if (opaque) *opaque = true;
+                            return true;
+                        }
                     }
                 }
                 continue;
@@ -690,8 +703,37 @@ template <int W> struct Bvh {
                 }
             }
         }
-        if (shadow) return false;
+        if (shadow && !transp) return false;
         return hit.prim != 0xffffffffu;
+    }
+
+    // Scene::GetAttenuation (core/scene.cpp:192-208) looping Accelerator::GetAttenuation
+    // (accel/accelerator.cpp:26-68). out = [r, g, b, surfaces passed]; at most `max_steps`
+    // surfaces are passed (the device library's limit).
+    void Attenuation(Ray ray, const float* transp, float out[4], std::vector<std::pair<const Node<W>*, float>>& stack,
+                     uint32_t max_steps) const {
+        float att[3] = {1.0f, 1.0f, 1.0f};
+        uint32_t steps = 0;
+        while (!(att[0] == 0.0f && att[1] == 0.0f && att[2] == 0.0f)) {
+            Hit hit;
+            bool opq = false;
+            if (!GetIntersect(ray, hit, true, stack, nullptr, transp, &opq)) break;
+            float a[3] = {0.0f, 0.0f, 0.0f};  // blocked: attenuation = 0
+            if (!opq && transp[4 * (size_t)hit.prim + 3] != 0.0f)
+                for (int c = 0; c < 3; c++) a[c] = transp[4 * (size_t)hit.prim + c];  // EvaluateTransparency
+            if (a[0] == 0.0f && a[1] == 0.0f && a[2] == 0.0f) {
+                out[0] = a[0]; out[1] = a[1]; out[2] = a[2]; out[3] = (float)steps;
+                return;
+            }
+            for (int c = 0; c < 3; c++) att[c] *= a[c];
+            steps++;
+            if (att[0] == 0.0f && att[1] == 0.0f && att[2] == 0.0f) break;
+            if (steps >= max_steps) break;
+            ray.o = V3(ray.o.x + hit.t * ray.d.x, ray.o.y + hit.t * ray.d.y, ray.o.z + hit.t * ray.d.z);  // ray(t)
+            ray.fmin = 0.001f;
+            ray.fmax = ray.fmax - hit.t;
+        }
+        out[0] = att[0]; out[1] = att[1]; out[2] = att[2]; out[3] = (float)steps;
     }
 };
 
@@ -976,6 +1018,31 @@ int oracle_ray1(OracleHandle* h, const void* ray_, void* hit_, int mode) {
     }
     if (mode == 0) *static_cast<Hit*>(hit_) = hh;
     return res ? 1 : 0;
+}
+
+// Transparent-shadow attenuation of n rays: transp = 4 floats per primitive id; out = 4 floats per
+// ray (r, g, b, surfaces passed). Walks stop after max_steps surfaces.
+void oracle_attenuation(OracleHandle* h, const void* rays_, uint64_t n, const float* transp, float* out,
+                        uint32_t max_steps, int nthreads) {
+    const RayRec* rays = static_cast<const RayRec*>(rays_);
+    std::atomic<uint64_t> next{0};
+    auto work = [&]() {
+        std::vector<std::pair<const Node<8>*, float>> s8;
+        std::vector<std::pair<const Node<4>*, float>> s4;
+        while (true) {
+            const uint64_t lo = next.fetch_add(1024);
+            if (lo >= n) break;
+            const uint64_t hi = std::min(n, lo + 1024);
+            for (uint64_t i = lo; i < hi; i++) {
+                if (h->width == 8) B<8>(h)->Attenuation(toRay(rays[i]), transp, out + 4 * i, s8, max_steps);
+                else B<4>(h)->Attenuation(toRay(rays[i]), transp, out + 4 * i, s4, max_steps);
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nthreads; w++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
 }
 
 // Brute force over all primitives (same per-triangle arithmetic). mode as above.

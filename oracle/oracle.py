@@ -40,6 +40,7 @@ def lib():
         l.oracle_trace.restype = ctypes.c_double
         l.oracle_trace.argtypes = [_P, _P, ctypes.c_uint64, _P, _P, ctypes.c_int, ctypes.c_int, _P]
         l.oracle_brute.argtypes = [_P, _P, ctypes.c_uint64, _P, _P, ctypes.c_int]
+        l.oracle_attenuation.argtypes = [_P, _P, ctypes.c_uint64, _P, _P, ctypes.c_uint32, ctypes.c_int]
         l.oracle_ray_setup.argtypes = [_P, _P, _P]
         l.oracle_tri_simd.restype = ctypes.c_int
         l.oracle_tri_simd.argtypes = [_P, _P, _P]
@@ -134,6 +135,15 @@ class OracleBvh:
             stats.update(dict(zip(["rays", "inner_nodes", "leaves", "tris", "quads"], [int(x) for x in st])))
             stats["seconds"] = secs
         return occ
+
+    def attenuation(self, rays, transparency, threads=1, max_steps=4096):
+        """Transparent-shadow attenuation (Scene::GetAttenuation): (n,4) [r, g, b, surfaces passed].
+        transparency: (n_prims, 4) per primitive id [r, g, b, 1 if the material has transparency]."""
+        rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 8))
+        tr = np.ascontiguousarray(np.asarray(transparency, np.float32).reshape(-1, 4))
+        out = np.zeros((len(rays), 4), np.float32)
+        lib().oracle_attenuation(self.h, _ptr(rays), len(rays), _ptr(tr), _ptr(out), max_steps, threads)
+        return out
 
     def brute_closest(self, rays):
         rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 8))

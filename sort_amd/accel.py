@@ -171,6 +171,26 @@ class Accelerator:
         check(lib().sortgpu_trace_any(self.handle, d_rays.data_ptr(), n, occ.data_ptr(), s), self.ctx.handle)
         return occ.cpu().numpy() if host else occ
 
+    def attenuation(self, rays, transparency, stream=None):
+        """Transparent-shadow attenuation (SORT's Scene::GetAttenuation): (n,4) float32 records
+        [r, g, b, surfaces passed]. `transparency`: (n_prims, 4) float32, per primitive id
+        (triangles then quads) the material's transparency r, g, b and 1.0 when the material has
+        transparency (else 0.0)."""
+        import torch
+        d_rays, host = self._device_rays(rays)
+        n = d_rays.shape[0]
+        if isinstance(transparency, torch.Tensor):
+            tr = transparency
+            assert tr.is_cuda and tr.dtype == torch.float32 and tr.is_contiguous()
+        else:
+            tr = torch.from_numpy(np.ascontiguousarray(np.asarray(transparency, np.float32).reshape(-1, 4))).to(
+                d_rays.device)
+        att = torch.empty((n, 4), dtype=torch.float32, device=d_rays.device)
+        s = stream if stream is not None else torch.cuda.current_stream(d_rays.device).cuda_stream
+        check(lib().sortgpu_trace_attenuation(self.handle, d_rays.data_ptr(), n, tr.data_ptr(), tr.numel() // 4,
+                                              att.data_ptr(), s), self.ctx.handle)
+        return att.cpu().numpy() if host else att
+
     def counters(self, rays, mode: int = 0):
         """(results, {inner_nodes, leaves, tris, quads}) for closest (0) or any-hit (1)."""
         import torch

@@ -364,6 +364,45 @@ int sortgpu_trace_any(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n,
     return trace_common(acc, d_rays, n, d_occluded, true, false, nullptr, stream);
 }
 
+int sortgpu_trace_attenuation(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, const float* d_transparency,
+                              uint32_t n_transparency, float* d_atten, void* stream) {
+    if (!acc) {
+        set_error(nullptr, "trace_attenuation: accel is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    sortgpu_ctx* ctx = acc->ctx;
+    if (ctx->host_only) {
+        set_error(ctx, "trace_attenuation: host-only context has no device");
+        return SORTGPU_ERR_STATE;
+    }
+    if (n == 0) return SORTGPU_OK;
+    if (!d_rays || !d_transparency || !d_atten) {
+        set_error(ctx, "trace_attenuation: null ray, transparency or output pointer");
+        return SORTGPU_ERR_ARG;
+    }
+    const uint64_t nprims = acc->idx.size() / 3 + acc->quads.size();
+    if (n_transparency < nprims) {
+        set_error(ctx, "trace_attenuation: %u transparency records for %llu primitives", n_transparency,
+                  (unsigned long long)nprims);
+        return SORTGPU_ERR_ARG;
+    }
+    int rc = ctx_bind(ctx);
+    if (rc != SORTGPU_OK) return rc;
+    TraceArgs a{};
+    a.tree = acc->view;
+    a.rays = d_rays;
+    a.n = n;
+    a.out = d_atten;
+    const uint32_t slot = acc->slot.fetch_add(1) % kCounterSlots;
+    a.counter = acc->d_counters + slot * kCounterStride;
+    a.error = acc->d_error;
+    a.transp = d_transparency;
+    apply_tuning(ctx, a);
+    hipError_t e = launch_attenuation(a, static_cast<hipStream_t>(stream), ctx->num_cus);
+    if (e != hipSuccess) return hip_fail(ctx, e, "attenuation launch");
+    return SORTGPU_OK;
+}
+
 int sortgpu_trace_counters(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, int mode,
                            sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4], void* stream) {
     if (!out || (mode != 0 && mode != 1)) {

@@ -285,12 +285,18 @@ struct Flattener {
 
     uint32_t leaf_ref(const BNode* n) { return REF_LEAF | n->off; }
 
-    // Leaf records: triangles (build order) then quads, last record flagged.
+    // Leaf records: triangles (build order) then quads, last record flagged. The triangles form
+    // groups of W in that order, the last one possibly partial (the reference packs a leaf's
+    // triangles into Simd_Triangle groups this way, fast_bvh.hpp:226-251); the last triangle of
+    // each group is flagged PR_GEND.
     void emit_leaf(const BNode* n) {
         uint32_t w = n->off;
+        uint32_t j = 0;  // triangles emitted so far in this leaf
         for (uint32_t i = n->off; i < n->off + n->cnt; i++) {
             const uint32_t id = B.P[i].id;
             if (id >= in.ntris) continue;
+            if (j > 0 && j % W == 0) prims[w - 1].flags |= PR_GEND;
+            j++;
             PrimRec& r = prims[w++];
             const uint32_t* t = in.idx + 3 * (size_t)id;
             const float* a = in.pos + 3 * (size_t)t[0];
@@ -303,6 +309,7 @@ struct Flattener {
             r.flags = 0;
             r.aux = 0;
         }
+        if (j > 0) prims[w - 1].flags |= PR_GEND;
         for (uint32_t i = n->off; i < n->off + n->cnt; i++) {
             const uint32_t id = B.P[i].id;
             if (id < in.ntris) continue;

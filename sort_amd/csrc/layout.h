@@ -25,6 +25,7 @@ constexpr uint32_t REF_LEAF = 0x80000000u;
 
 constexpr uint32_t PR_LAST = 1u;  // last record of the leaf
 constexpr uint32_t PR_QUAD = 2u;  // record is a quad; `prim` field holds the quad index
+constexpr uint32_t PR_GEND = 4u;  // last triangle of a group of W (the reference's Simd_Triangle)
 
 template <int W> struct alignas(W == 8 ? 256 : 128) NodeRec {
     float lo_x[W], hi_x[W], lo_y[W], hi_y[W], lo_z[W], hi_z[W];

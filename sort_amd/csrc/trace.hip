@@ -42,11 +42,16 @@ constexpr uint32_t kDefaultLeafBatch = 32;
 // One primitive record of a lane's leaf: `pr` is tested while the leaf's next record (if any) is
 // loaded into `nxt`. Closest hit keeps the first record with the smallest t in (tmin, tmax]
 // (strict t < bt for triangles; a quad at the current t replaces it); any hit ends the ray.
-template <bool ANY, bool STATS>
+// Transparent-shadow walk (TRANSP): a closest-hit walk that stops, blocked, when a group of W
+// triangles (PR_GEND ends one) or a quad makes the closest hit and that primitive's material has no
+// transparency (fast_bvh.hpp:352-373 and 394-405, the ENABLE_TRANSPARENT_SHADOW branches: only the
+// group's nearest hit is checked). `gup`: the current group has made the closest hit.
+template <bool ANY, bool STATS, bool TRANSP>
 __device__ __forceinline__ void leaf_step(const RayK& r, const PrimRec* __restrict__ prims,
-                                          const QuadRec* __restrict__ quads, const PrimRec& pr, PrimRec& nxt,
-                                          uint32_t& li, int& sp, float& bt, uint32_t& bp, float& bu, float& bv,
-                                          bool& occluded, uint32_t& n_tris, uint32_t& n_quads) {
+                                          const QuadRec* __restrict__ quads, const float* __restrict__ transp,
+                                          const PrimRec& pr, PrimRec& nxt, uint32_t& li, int& sp, float& bt,
+                                          uint32_t& bp, float& bu, float& bv, bool& occluded, bool& gup,
+                                          uint32_t& n_tris, uint32_t& n_quads) {
     if (li == 0xffffffffu) return;
     if (!(pr.flags & PR_LAST)) nxt = load_prim(prims, li + 1);
     float t, u, v;
@@ -69,7 +74,19 @@ __device__ __forceinline__ void leaf_step(const RayK& r, const PrimRec* __restri
             hit = true;
         }
     }
-    if (ANY && hit) {
+    bool blocked = false;
+    if (TRANSP) {
+        if (pr.flags & PR_QUAD) {
+            blocked = hit && transp[4 * (size_t)bp + 3] == 0.0f;
+        } else {
+            gup |= hit;
+            if (pr.flags & PR_GEND) {
+                blocked = gup && transp[4 * (size_t)bp + 3] == 0.0f;
+                gup = false;
+            }
+        }
+    }
+    if ((ANY && hit) || blocked) {
         occluded = true;
         li = 0xffffffffu;
         sp = 0;
@@ -78,7 +95,7 @@ __device__ __forceinline__ void leaf_step(const RayK& r, const PrimRec* __restri
     }
 }
 
-template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS>
+template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS, bool TRANSP = false>
 // At least 3 waves per SIMD (at most 168 VGPRs): the occupancy every W = 8 form is tuned for.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_trace(TraceArgs a) {
     // stack entry: (child reference, entry distance bits)
@@ -95,6 +112,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const NodeRec<W>* __restrict__ nodes = static_cast<const NodeRec<W>*>(a.tree.nodes);
     const PrimRec* __restrict__ prims = a.tree.prims;
     const QuadRec* __restrict__ quads = a.tree.quads;
+    const float* __restrict__ transp = a.transp;
 
     RayK r;
     uint32_t li = 0xffffffffu;  // next primitive record of this lane's leaf
@@ -119,6 +137,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     };
 
     bool occluded = false;
+    // transparent-shadow walk: the current group made the closest hit; attenuation so far; surfaces passed
+    bool gup = false;
+    float att_r = 1.0f, att_g = 1.0f, att_b = 1.0f;
+    uint32_t steps = 0;
     // Whether the leaf phase takes another step. It may stop while leaves are unfinished only
     // when fewer than `leaf_min` lanes hold one and the next node phase can take a step (some lane
     // can walk and fewer than `leaf_batch` lanes hold a leaf), so every round advances some lane
@@ -154,6 +176,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                     bt = FLT_MAX;
                     bp = 0xffffffffu;
                     bu = bv = 0.f;
+                    if (TRANSP) {
+                        att_r = att_g = att_b = 1.0f;
+                        steps = 0;
+                        gup = false;
+                    }
                     uint32_t root, valid;
                     float rmin = -1.0f;
                     if (MULTI) {
@@ -182,7 +209,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                         push(0, root, rmin);
                         sp = 1;
                     } else {
-                        if (ANY) static_cast<uint8_t*>(a.out)[my] = 0;
+                        if (TRANSP) reinterpret_cast<float4*>(a.out)[my] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                        else if (ANY) static_cast<uint8_t*>(a.out)[my] = 0;
                         else reinterpret_cast<float4*>(a.out)[my] =
                                  make_float4(FLT_MAX, __uint_as_float(0xffffffffu), 0.f, 0.f);
                         ray_id = 0xffffffffu;
@@ -265,15 +293,68 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         // record under test and the one loading swap roles without register copies.
         while (true) {
             if (!leaf_go()) break;
-            leaf_step<ANY, STATS>(r, prims, quads, cur, alt, li, sp, bt, bp, bu, bv, occluded, n_tris, n_quads);
+            leaf_step<ANY, STATS, TRANSP>(r, prims, quads, transp, cur, alt, li, sp, bt, bp, bu, bv, occluded, gup, n_tris,
+                                          n_quads);
             if (!leaf_go()) {
                 cur = alt;
                 break;
             }
-            leaf_step<ANY, STATS>(r, prims, quads, alt, cur, li, sp, bt, bp, bu, bv, occluded, n_tris, n_quads);
+            leaf_step<ANY, STATS, TRANSP>(r, prims, quads, transp, alt, cur, li, sp, bt, bp, bu, bv, occluded, gup, n_tris,
+                                          n_quads);
         }
         if (occluded) sp = 0;
-        if (ray_id != 0xffffffffu && sp == 0 && li == 0xffffffffu) {
+        if (TRANSP && ray_id != 0xffffffffu && sp == 0 && li == 0xffffffffu) {
+            // Scene::GetAttenuation (scene.cpp:192-208) around Accelerator::GetAttenuation
+            // (accelerator.cpp:26-68): blocked -> black; no hit -> the attenuation so far;
+            // else multiply by the hit material's transparency (black without a transparent node)
+            // and walk on from the hit point with (0.001, tmax - t).
+            bool done = true;
+            if (occluded) {
+                att_r = att_g = att_b = 0.0f;
+            } else if (bp != 0xffffffffu) {
+                const float4 tr = reinterpret_cast<const float4*>(transp)[bp];
+                const bool has = tr.w != 0.0f;
+                const float ar = has ? tr.x : 0.0f, ag = has ? tr.y : 0.0f, ab = has ? tr.z : 0.0f;
+                if (ar == 0.0f && ag == 0.0f && ab == 0.0f) {
+                    att_r = ar;
+                    att_g = ag;
+                    att_b = ab;
+                } else {
+                    att_r *= ar;
+                    att_g *= ag;
+                    att_b *= ab;
+                    steps++;
+                    if (!(att_r == 0.0f && att_g == 0.0f && att_b == 0.0f)) {
+                        if (steps >= kMaxAttenuationSteps) {
+                            atomicOr(a.error, 4u);
+                        } else {
+                            r.ox = r.ox + bt * r.dx;  // ray(t) = o + t * d
+                            r.oy = r.oy + bt * r.dy;
+                            r.oz = r.oz + bt * r.dz;
+                            r.tmin = 0.001f;
+                            r.tmax = r.tmax - bt;
+                            ray_setup(r);
+                            r.ieee = ieee_slabs_ok(r, a.tree.bbox);
+                            const float rmin = a.tree.valid ? root_slab(r, a.tree.bbox) : -1.0f;
+                            if (rmin >= 0.0f) {
+                                push(0, a.tree.root, rmin);
+                                sp = 1;
+                                bt = FLT_MAX;
+                                bp = 0xffffffffu;
+                                bu = bv = 0.f;
+                                gup = false;
+                                done = false;
+                            }
+                        }
+                    }
+                }
+            }
+            if (done) {
+                reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(att_r, att_g, att_b, (float)steps);
+                ray_id = 0xffffffffu;
+            }
+        }
+        if (!TRANSP && ray_id != 0xffffffffu && sp == 0 && li == 0xffffffffu) {
             if (ANY) static_cast<uint8_t*>(a.out)[ray_id] = occluded ? 1 : 0;
             else reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);
             ray_id = 0xffffffffu;
@@ -297,9 +378,9 @@ __global__ void k_reset(uint32_t* counter) {
     if (threadIdx.x == 0) *counter = 0;
 }
 
-template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS>
+template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS, bool TRANSP = false>
 static hipError_t launch_one(const TraceArgs& a, hipStream_t stream, int num_cus) {
-    auto kern = k_trace<W, ANY, STATS, SPILL, MULTI, LDS>;
+    auto kern = k_trace<W, ANY, STATS, SPILL, MULTI, LDS, TRANSP>;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0);
     if (e != hipSuccess) return e;
@@ -343,6 +424,20 @@ hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t st
     if (width == 8) return multi ? launch_w<8, true>(a, any, stats, stream, num_cus)
                                  : launch_w<8, false>(a, any, stats, stream, num_cus);
     return multi ? launch_w<4, true>(a, any, stats, stream, num_cus) : launch_w<4, false>(a, any, stats, stream, num_cus);
+}
+
+template <int W>
+static hipError_t launch_att_w(const TraceArgs& a, hipStream_t stream, int num_cus) {
+    const uint32_t need = a.tree.max_stack;
+    if (need <= 16u + 48u) return launch_one<W, false, false, 48, false, 16, true>(a, stream, num_cus);
+    if (need <= 16u + 240u) return launch_one<W, false, false, 240, false, 16, true>(a, stream, num_cus);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_attenuation(const TraceArgs& a, hipStream_t stream, int num_cus) {
+    if (a.n == 0) return hipSuccess;
+    if (!a.transp || a.trees != nullptr || a.order != nullptr) return hipErrorInvalidValue;
+    return a.tree.width == 8 ? launch_att_w<8>(a, stream, num_cus) : launch_att_w<4>(a, stream, num_cus);
 }
 
 uint32_t trace_max_stack_supported() { return 8 + 240; }

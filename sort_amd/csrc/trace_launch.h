@@ -31,10 +31,18 @@ struct TraceArgs {
     // optional traversal order: the k-th ray taken from the work counter is rays[order[k]] (a
     // permutation of 0..n-1, device), its result goes to out[order[k]]; null = index order.
     const uint32_t* order = nullptr;
+    // transparent-shadow attenuation only: per primitive id, 4 floats (transparency r, g, b, and
+    // 1 when the primitive's material has transparency, else 0)
+    const float* transp = nullptr;
 };
 
 // Launches the closest (any = false) or any-hit traversal on `stream`.
 hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t stream, int num_cus);
+// Launches the transparent-shadow attenuation walk (single tree, a.transp set): out receives one
+// float4 per ray (attenuation r, g, b, and the number of surfaces passed as a float).
+hipError_t launch_attenuation(const TraceArgs& a, hipStream_t stream, int num_cus);
+// Most surfaces one attenuation walk passes before it gives up (error bit 2).
+constexpr uint32_t kMaxAttenuationSteps = 4096;
 
 // Traversal order of a ray batch (order.hip): ray indices sorted by the origin's cell in the tree
 // box, so that a wave's lanes walk nearby nodes. Buffers come from `pool`, stream-ordered on `s`;
