@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tris", type=int, default=1 << 20)
-    p.add_argument("--legs", default="intersection,pt,batched,served")
+    p.add_argument("--legs", default="intersection,attenuation,pt,batched,served")
     p.add_argument("--served-requests", type=int, default=60)
     p.add_argument("--copy-probe", action="store_true",
                    help="end with three 256 MiB device copies (HBM counter calibration)")
@@ -169,6 +169,47 @@ def leg_batched(ctx):
     return res
 
 
+def leg_attenuation(acc, d_sh, sh, pos, idx, stream, a, cpu=True):
+    """Transparent shadows (Scene::GetAttenuation): the 4M area-light shadow segments through the
+    soup with a seeded table in which half of the triangles' materials are transparent (colour in
+    [0.2, 1)), the rest opaque. GPU rate over the whole batch; the oracle on 1M of the segments."""
+    import torch
+    from sort_amd._lib import check, lib as _l
+    n = d_sh.shape[0]
+    g = np.random.default_rng(2024)
+    tr = np.zeros((len(idx), 4), np.float32)
+    tr[:, :3] = g.uniform(0.2, 1.0, (len(idx), 3)).astype(np.float32)
+    tr[:, 3] = (g.random(len(idx)) < 0.5).astype(np.float32)
+    d_tr = torch.from_numpy(tr).to(d_sh.device)
+    out = torch.empty((n, 4), dtype=torch.float32, device=d_sh.device)
+    lib = _l()
+
+    def run():
+        check(lib.sortgpu_trace_attenuation(acc.handle, d_sh.data_ptr(), n, d_tr.data_ptr(), len(idx),
+                                            out.data_ptr(), stream.cuda_stream))
+
+    _, ms, _ = timed(run, max(5, a.steps // 5), 2, stream)
+    res = out.cpu().numpy()
+    rec = {"workload": f"{n} area-light shadow segments, 50% transparent materials",
+           "mrays_s": round(n / (ms * 1e-3) / 1e6, 3), "launch_ms_mean": round(ms, 4),
+           "surfaces_per_ray": round(float(res[:, 3].mean()), 4),
+           "blocked_frac": round(float(np.all(res[:, :3] == 0, axis=1).mean()), 4)}
+    if cpu:
+        from oracle.oracle import OracleBvh
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        ob = OracleBvh(pos, idx, width=a.width)
+        sample = np.ascontiguousarray(sh[: 1 << 20])
+        ob.attenuation(sample[: 1 << 14], tr, threads=threads)  # warm-up
+        t0 = time.perf_counter()
+        ref = ob.attenuation(sample, tr, threads=threads)
+        secs = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(len(sample) / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "sample": f"first {len(sample)} segments, one run after a warm-up"}
+        rec["gpu_over_cpu"] = round(rec["mrays_s"] / rec["cpu_baseline"]["value"], 2)
+        rec["sample_bitwise_equal"] = bool(np.array_equal(ref.view(np.uint32), res[: len(sample)].view(np.uint32)))
+    return rec
+
+
 def memory_pipeline(n, st, mean_ms, value, stream, width):
     """The ceiling the traversal runs against: each lane reads its own records (a node as 14
     consecutive 16-byte chunks, a triangle record as 3), so a launch is bound by the rate at which
@@ -280,6 +321,9 @@ def main():
     inc_mrays = n * a.steps / wall / 1e6
     _, cam_ms, _ = timed(trace_cam, max(5, a.steps // 5), 2, stream)
     _, sh_ms, _ = timed(trace_sh, max(5, a.steps // 5), 2, stream)
+    att = None
+    if "attenuation" in set(a.legs.split(",")):
+        att = leg_attenuation(acc, d_sh, sh, pos, idx, stream, a, cpu=not a.no_cpu)
 
     b_inc = bytes_per_ray(st_inc, 16)
     achieved = b_inc * n / (mean_ms * 1e-3) / 1e9
@@ -337,6 +381,8 @@ def main():
     if cpu:
         out["config"]["gpu_over_cpu"] = round(inc_mrays / cpu["value"], 2)
     out["roofline"]["memory_pipeline"] = memory_pipeline(n, st_inc, mean_ms, inc_mrays, stream, a.width)
+    if att:
+        out["attenuation"] = att
     legs = set(a.legs.split(","))
     del d_inc, d_sh, d_cam, hits, occ
     acc.release()

@@ -286,6 +286,20 @@ static hipMemPool_t order_pool(sortgpu_ctx* ctx) {
     return ctx->order_pool;
 }
 
+// Coherence order for a large batch on `s` (the tuning's reorder knob decides), or null for index
+// order. An order that cannot be had (no pool, no memory) only costs speed, so it is not an error.
+// Results never depend on it; `ro` is released on `s` after the launch that reads it.
+static const uint32_t* traversal_order(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, hipStream_t s,
+                                       RayOrder& ro) {
+    sortgpu_ctx* ctx = acc->ctx;
+    const uint32_t reorder = ctx->tuning.reorder;
+    if (reorder == 1 || (reorder != 2 && n < kReorderMinRays) || n > 0x7fffffffu) return nullptr;
+    hipMemPool_t pool = order_pool(ctx);
+    if (pool && order_rays(pool, d_rays, n, acc->view.bbox, s, ro) == hipSuccess) return ro.order;
+    (void)hipGetLastError();
+    return nullptr;
+}
+
 static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, void* d_out, bool any,
                         bool stats, uint64_t* out4, void* stream) {
     if (!acc) {
@@ -326,13 +340,7 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     }
     // traversal order (results do not depend on it; the counter launch keeps index order)
     RayOrder ro;
-    const uint32_t reorder = ctx->tuning.reorder;
-    if (!stats && reorder != 1 && (reorder == 2 || n >= kReorderMinRays) && n <= 0x7fffffffu) {
-        hipMemPool_t pool = order_pool(ctx);
-        // an order that cannot be had (no pool, no memory) only costs speed: trace in index order
-        if (pool && order_rays(pool, d_rays, n, acc->view.bbox, s, ro) == hipSuccess) a.order = ro.order;
-        else (void)hipGetLastError();
-    }
+    if (!stats) a.order = traversal_order(acc, d_rays, n, s, ro);
     hipError_t e = launch_trace(a, any, stats, s, ctx->num_cus);
     if (ro.order) release_order(ro, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
@@ -398,7 +406,11 @@ int sortgpu_trace_attenuation(sortgpu_accel* acc, const sortgpu_ray* d_rays, uin
     a.error = acc->d_error;
     a.transp = d_transparency;
     apply_tuning(ctx, a);
-    hipError_t e = launch_attenuation(a, static_cast<hipStream_t>(stream), ctx->num_cus);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    RayOrder ro;
+    a.order = traversal_order(acc, d_rays, n, s, ro);
+    hipError_t e = launch_attenuation(a, s, ctx->num_cus);
+    if (ro.order) release_order(ro, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "attenuation launch");
     return SORTGPU_OK;
 }

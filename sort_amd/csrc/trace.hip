@@ -436,7 +436,7 @@ static hipError_t launch_att_w(const TraceArgs& a, hipStream_t stream, int num_c
 
 hipError_t launch_attenuation(const TraceArgs& a, hipStream_t stream, int num_cus) {
     if (a.n == 0) return hipSuccess;
-    if (!a.transp || a.trees != nullptr || a.order != nullptr) return hipErrorInvalidValue;
+    if (!a.transp || a.trees != nullptr) return hipErrorInvalidValue;
     return a.tree.width == 8 ? launch_att_w<8>(a, stream, num_cus) : launch_att_w<4>(a, stream, num_cus);
 }
 

@@ -110,3 +110,25 @@ def test_attenuation_edge_cases(gpu_ctx):
     s.synchronize()
     assert np.array_equal(got.cpu().numpy().view(np.uint32),
                           ob.attenuation(rays, d_tr.cpu().numpy()).view(np.uint32))
+
+
+def test_attenuation_order_changes_no_result(gpu_ctx):
+    """Batches of 2^18 rays or more are walked in origin-cell order by default; never (1), always
+    (2) and the default (0) give the same bits, equal to the oracle's."""
+    from sort_amd import batch
+    pos, idx = _dense_soup(1 << 14, 95)
+    acc, ob = _pair(gpu_ctx, pos, idx)
+    rays = scenes.random_rays(300_000, 96, lo=-0.2, hi=1.2)
+    tr = _table(len(idx), 97)
+    d_rays, d_tr = torch.from_numpy(rays).cuda(), torch.from_numpy(tr).cuda()
+    res = {}
+    try:
+        for mode in (1, 2, 0):
+            batch.set_tuning(gpu_ctx, reorder=mode)
+            print(f"[atten order] mode {mode}", flush=True)
+            res[mode] = acc.attenuation(d_rays, d_tr).cpu().numpy()
+    finally:
+        batch.set_tuning(gpu_ctx)
+    for mode in (2, 0):
+        assert np.array_equal(res[mode].view(np.uint32), res[1].view(np.uint32)), mode
+    assert np.array_equal(res[0].view(np.uint32), ob.attenuation(rays, tr, threads=16).view(np.uint32))
