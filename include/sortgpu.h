@@ -336,6 +336,79 @@ int sortgpu_server_drain(sortgpu_server* srv, double timeout_s);
  * [11] cancelled, [12] timed out, [13] device busy microseconds, [14] budget, [15] workspace bytes. */
 int sortgpu_server_stats(sortgpu_server* srv, uint64_t out[16]);
 
+/* ---- scene files: the .sort stream (host only; no device is touched) ----
+ * One file = the global configuration (ImageEvaluation::loadConfig,
+ * work/image_evaluation/image_evaluation.cpp:354-378), the material section
+ * (MatManager::ParseMatFile, material/matmanager.cpp:50-290, Material::Serialize,
+ * material/material.cpp:153-228) and the entity stream (Scene::LoadScene, core/scene.cpp:112-180),
+ * as blender-plugin/addons/sortblend/exporter.py writes it. The loader reads every record of the
+ * stream the way those readers do and flattens the scene into the arrays the accelerator and the
+ * path tracer take. */
+
+#define SORTGPU_ERR_FORMAT (-6) /* unreadable or malformed scene file (message names the offset) */
+
+/* Content of a file that the GPU path does not carry: bits of sortgpu_scene_file_info.skipped;
+ * counts[bit] = how many items. */
+#define SORTGPU_FILE_PUNCTUAL_LIGHTS 0  /* point, spot and directional lights (light_entity.cpp:25-67) */
+#define SORTGPU_FILE_DISK_LIGHTS 1      /* disk area lights (light_entity.cpp:137-148)                  */
+#define SORTGPU_FILE_HAIR 2             /* HairVisual curves (entity/visual.cpp:44-100)                */
+#define SORTGPU_FILE_VOLUMES 3          /* mesh density grids and volume shaders                       */
+#define SORTGPU_FILE_SKY_IMAGE 4        /* HDR sky image (the constant sky colour is kept)             */
+#define SORTGPU_FILE_UNMAPPED_MATS 5    /* materials whose shader graph has no GPU BXDF: white Lambert */
+#define SORTGPU_FILE_APPROX_MATS 6      /* materials mapped from their constant inputs only (a texture
+                                           or other node feeds an input, anisotropic roughness, a
+                                           non-GGX distribution, transparency, SSS)                   */
+#define SORTGPU_FILE_CAMERA 7           /* lens radius (depth of field) or non-square pixels            */
+
+typedef struct sortgpu_scene_file sortgpu_scene_file;
+
+typedef struct sortgpu_scene_file_info {
+    /* global configuration */
+    uint32_t threads, spp, image_width, image_height;
+    float clamping;
+    uint32_t integrator;           /* SID (CRC, core/strid.h:39-60) of the integrator class name   */
+    int32_t max_recursive_depth;   /* Integrator::Serialize (integrator.h:85-87)                     */
+    int32_t max_bssrdf_bounces;    /* PathTracing only (pathtracing.h:42-45), else 0                 */
+    /* camera as stored (PerspectiveCameraEntity, camera_entity.cpp:21-32) */
+    float eye[3], up[3], target[3];
+    uint32_t camera_width, camera_height;
+    float lens_radius, sensor_w, sensor_h;
+    int32_t aspect_fit;
+    float pixel_aspect_w, pixel_aspect_h, fov;
+    /* the pinhole camera the renderer takes: eye, target, up, and the vertical field of view
+     * equal to PerspectiveCamera::PreProcess's (camera/perspective.cpp:26-63) for this sensor fit */
+    float camera_fov_y;
+    uint32_t has_camera;
+    /* geometry in primitive-id order: the triangles of every mesh entity in file order (world
+     * space, Mesh::ApplyTransform, core/mesh.cpp:26-40), then one quad per square or rectangular
+     * area light in file order (AreaLightEntity::Serialize, light_entity.cpp:99-155) */
+    uint32_t nverts, ntris, nquads, nmaterials;
+    const float* pos;              /* nverts x 3 */
+    const float* normals;          /* nverts x 3 (TransformNormal of the normalised normal) */
+    const float* uv;               /* nverts x 2 */
+    const uint32_t* idx;           /* ntris x 3 */
+    const uint32_t* tri_material;  /* ntris: index into materials (a mesh's -1 or unknown id: the last entry) */
+    const sortgpu_quad* quads;     /* nquads */
+    const float* quad_radiance;    /* nquads x 3 */
+    const sortgpu_material* materials; /* the file's materials in order, then SORT's default white Lambert */
+    float env_radiance[3];         /* sky colour x intensity; 0.5 when the file has no light at all */
+    /* accelerator record after the entities */
+    uint32_t accelerator;          /* SID of the accelerator class name */
+    int32_t accel_max_depth, accel_max_leaf;
+    uint32_t skipped;              /* bits 1 << SORTGPU_FILE_* */
+    uint32_t counts[8];
+} sortgpu_scene_file_info;
+
+/* Reads a .sort file (or an in-memory copy of one). Every read is bounds-checked: a short,
+ * corrupt or unknown-record stream is refused with SORTGPU_ERR_FORMAT and a message naming the
+ * byte offset, where SORT would assert. The arrays stay valid until sortgpu_scene_file_destroy. */
+int sortgpu_scene_file_load(const char* path, sortgpu_scene_file** out);
+int sortgpu_scene_file_parse(const void* data, uint64_t size, sortgpu_scene_file** out);
+const sortgpu_scene_file_info* sortgpu_scene_file_get(const sortgpu_scene_file* file);
+/* Name of material i (< nmaterials - 1; the last, default material is "default"). */
+const char* sortgpu_scene_file_material_name(const sortgpu_scene_file* file, uint32_t i);
+void sortgpu_scene_file_destroy(sortgpu_scene_file* file);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,24 @@ class Camera(ctypes.Structure):
                 ("fov_y", ctypes.c_float)]
 
 
+class SceneFileInfo(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_uint32), ("spp", ctypes.c_uint32), ("image_width", ctypes.c_uint32),
+                ("image_height", ctypes.c_uint32), ("clamping", ctypes.c_float), ("integrator", ctypes.c_uint32),
+                ("max_recursive_depth", ctypes.c_int32), ("max_bssrdf_bounces", ctypes.c_int32),
+                ("eye", ctypes.c_float * 3), ("up", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
+                ("camera_width", ctypes.c_uint32), ("camera_height", ctypes.c_uint32),
+                ("lens_radius", ctypes.c_float), ("sensor_w", ctypes.c_float), ("sensor_h", ctypes.c_float),
+                ("aspect_fit", ctypes.c_int32), ("pixel_aspect_w", ctypes.c_float), ("pixel_aspect_h", ctypes.c_float),
+                ("fov", ctypes.c_float), ("camera_fov_y", ctypes.c_float), ("has_camera", ctypes.c_uint32),
+                ("nverts", ctypes.c_uint32), ("ntris", ctypes.c_uint32), ("nquads", ctypes.c_uint32),
+                ("nmaterials", ctypes.c_uint32), ("pos", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+                ("uv", ctypes.c_void_p), ("idx", ctypes.c_void_p), ("tri_material", ctypes.c_void_p),
+                ("quads", ctypes.c_void_p), ("quad_radiance", ctypes.c_void_p), ("materials", ctypes.c_void_p),
+                ("env_radiance", ctypes.c_float * 3), ("accelerator", ctypes.c_uint32),
+                ("accel_max_depth", ctypes.c_int32), ("accel_max_leaf", ctypes.c_int32),
+                ("skipped", ctypes.c_uint32), ("counts", ctypes.c_uint32 * 8)]
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("wave_paths", ctypes.c_uint32), ("max_active", ctypes.c_uint32),
                 ("inflight_waves", ctypes.c_uint32), ("manual", ctypes.c_uint32), ("host_device", ctypes.c_uint32),
@@ -127,6 +145,11 @@ _SIGS = {
     "sortgpu_server_step": (_I, [_P, ctypes.POINTER(_I)]),
     "sortgpu_server_drain": (_I, [_P, ctypes.c_double]),
     "sortgpu_server_stats": (_I, [_P, _P]),
+    "sortgpu_scene_file_load": (_I, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "sortgpu_scene_file_parse": (_I, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "sortgpu_scene_file_get": (ctypes.POINTER(SceneFileInfo), [_P]),
+    "sortgpu_scene_file_material_name": (ctypes.c_char_p, [_P, _U32]),
+    "sortgpu_scene_file_destroy": (None, [_P]),
 }
 
 
