@@ -115,6 +115,20 @@ int sortgpu_trace_closest(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint3
 int sortgpu_trace_any(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
                       uint8_t* d_occluded, void* stream);
 
+/* BSSRDF probe query (Accelerator::GetIntersect with BSSRDFIntersections and a material id,
+ * accel/fast_bvh.hpp:656-752, simd/simd_triangle.h:412-452, scatteringevent/bssrdf/bssrdf.h:32-56):
+ * up to 4 triangle hits per ray whose triangle has material d_ray_material[i]
+ * (d_tri_material[prim] per triangle id; quads are never reported). d_hits[4i..4i+3] are the four
+ * records in the reference's slot order: filled in the order the walk finds them, then each new
+ * hit replaces the first record with the largest t; a hit counts when t is in (tmin, tmax] and
+ * below the largest recorded t as it stood before its group of W leaf triangles (that bound is
+ * only tightened once a record has been replaced, as in SORT). Unused records are (FLT_MAX,
+ * 0xFFFFFFFF, 0, 0); d_counts[i] = records filled (0..4). n_tri_material must cover every
+ * triangle (else SORTGPU_ERR_ARG). Stream-ordered. */
+int sortgpu_trace_multi(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n, const uint32_t* d_ray_material,
+                        const uint32_t* d_tri_material, uint32_t n_tri_material, sortgpu_hit* d_hits,
+                        uint32_t* d_counts, void* stream);
+
 /* Transparent-shadow attenuation for n rays (Scene::GetAttenuation, core/scene.cpp:192-208,
  * looping Accelerator::GetAttenuation, accel/accelerator.cpp:26-68, with the transparent-shadow
  * walk of fast_bvh.hpp:352-405). d_transparency: 4 floats per primitive id (triangles, then quads):

@@ -331,6 +331,14 @@ struct Scene {
     BBox bbox;               // enlarged scene bbox
 };
 
+// BSSRDFIntersections (scatteringevent/bssrdf/bssrdf.h:32-56): at most four records, maxt only
+// resolved when a record is replaced.
+struct MultiHits {
+    Hit rec[4];
+    uint32_t cnt = 0;
+    float maxt = FLT_MAX;
+};
+
 template <int W> struct Bvh {
     Scene sc;
     std::vector<Prim> prims;
@@ -550,6 +558,54 @@ template <int W> struct Bvh {
         return S::movemask(mask);
     }
 
+    // simd_triangle.h:212-300 (quick_quit = false) alone: every lane's hit in (fmin, fmax] as a
+    // mask, with t (FLT_MAX outside the mask), u and v per lane.
+    static inline int TriLanes(const Ray& ray, const RayW<W>& sr, const TriW<W>& tri, float* tf, float* uf, float* vf) {
+        using S = Simd<W>;
+        using T = typename S::T;
+        T mask = S::load(tri.mask);
+        T p[3][3];
+        for (int v = 0; v < 3; v++)
+            for (int c = 0; c < 3; c++) p[v][c] = S::sub(S::load(tri.p[3 * v + c]), sr.ori[c]);
+        T* p0x = &p[0][ray.kx]; T* p0y = &p[0][ray.ky]; T* p0z = &p[0][ray.kz];
+        T* p1x = &p[1][ray.kx]; T* p1y = &p[1][ray.ky]; T* p1z = &p[1][ray.kz];
+        T* p2x = &p[2][ray.kx]; T* p2y = &p[2][ray.ky]; T* p2z = &p[2][ray.kz];
+        *p0x = S::mad(*p0y, sr.scale[0], *p0x);
+        *p0z = S::mad(*p0y, sr.scale[2], *p0z);
+        *p1x = S::mad(*p1y, sr.scale[0], *p1x);
+        *p1z = S::mad(*p1y, sr.scale[2], *p1z);
+        *p2x = S::mad(*p2y, sr.scale[0], *p2x);
+        *p2z = S::mad(*p2y, sr.scale[2], *p2z);
+        const T e0 = S::sub(S::mul(*p1x, *p2z), S::mul(*p1z, *p2x));
+        const T e1 = S::sub(S::mul(*p2x, *p0z), S::mul(*p2z, *p0x));
+        const T e2 = S::sub(S::mul(*p0x, *p1z), S::mul(*p0z, *p1x));
+        const T zeros = S::set1(0.0f);
+        const T c0 = S::and_(S::and_(S::cge(e0, zeros), S::cge(e1, zeros)), S::cge(e2, zeros));
+        const T c1 = S::and_(S::and_(S::cle(e0, zeros), S::cle(e1, zeros)), S::cle(e2, zeros));
+        mask = S::or_(S::and_(mask, c0), S::and_(mask, c1));
+        if (0 == S::movemask(mask)) return 0;
+        const T det = S::add(e0, S::add(e1, e2));
+        mask = S::and_(mask, S::cneq(det, zeros));
+        if (0 == S::movemask(mask)) return 0;
+        const T rcp_det = S::rcp(det);
+        *p0y = S::mul(*p0y, sr.scale[1]);
+        *p1y = S::mul(*p1y, sr.scale[1]);
+        *p2y = S::mul(*p2y, sr.scale[1]);
+        T t = S::mul(e0, *p0y);
+        t = S::mad(e1, *p1y, t);
+        t = S::mad(e2, *p2y, t);
+        t = S::mul(t, rcp_det);
+        mask = S::and_(S::and_(mask, S::cgt(t, S::set1(ray.fmin))), S::cle(t, S::set1(ray.fmax)));
+        if (0 == S::movemask(mask)) return 0;
+        t = S::pick(mask, t, S::set1(FLT_MAX));
+        const T u = S::mul(e1, rcp_det);
+        const T v = S::mul(e2, rcp_det);
+        S::store(tf, t);
+        S::store(uf, u);
+        S::store(vf, v);
+        return S::movemask(mask);
+    }
+
     // simd_triangle.h:212-300 (quick_quit = false) + 350-383. Returns true when a closer hit was
     // written into `hit`.
     static inline bool IntersectTri(const Ray& ray, const RayW<W>& sr, const TriW<W>& tri, Hit& hit) {
@@ -705,6 +761,107 @@ template <int W> struct Bvh {
         }
         if (shadow && !transp) return false;
         return hit.prim != 0xffffffffu;
+    }
+
+    // Fbvh::GetIntersect(ray, BSSRDFIntersections&, rc, matID), fast_bvh.hpp:656-752 (SIMD branch):
+    // the closest-hit walk pruned by maxt; each leaf's triangle groups through
+    // intersectTriangleMulti_SIMD (simd_triangle.h:412-452): the lanes hit in the ray's window with
+    // t < maxt as it was before the group, in lane order; those whose primitive has material `mat`
+    // fill the four records, then replace the first record with the largest t and resolve maxt.
+    // Quads are not tested (only triangles carry SSS).
+    void GetIntersectMulti(Ray ray, const uint32_t* tri_mat, uint32_t mat, MultiHits& out,
+                           std::vector<std::pair<const Node<W>*, float>>& stack) const {
+        out.cnt = 0;
+        out.maxt = FLT_MAX;
+        for (auto& r : out.rec) r = Hit{FLT_MAX, 0xffffffffu, 0.f, 0.f};
+        if (!valid) return;
+        ray.Prepare();
+        RayW<W> sr;
+        resolveRayData<W>(ray, sr);
+        const float rmin = IntersectScalar(ray, sc.bbox);
+        if (rmin < 0.0f) return;
+        if (stack.size() < (size_t)depth * W + W) stack.resize((size_t)depth * W + W);
+        int si = 0;
+        stack[si++] = std::make_pair(root.get(), rmin);
+        while (si > 0) {
+            const auto top = stack[--si];
+            const Node<W>* node = top.first;
+            const float fmin = top.second;
+            if (out.maxt < fmin) continue;
+            if (0 == node->child_cnt) {
+                for (const auto& g : node->tris) {
+                    alignas(32) float tf[W], uf[W], vf[W];
+                    int m = TriLanes(ray, sr, g, tf, uf, vf);
+                    if (!m) continue;
+                    int lt = 0;
+                    for (int l = 0; l < W; l++) lt |= (tf[l] < out.maxt) ? (1 << l) : 0;
+                    m &= lt;
+                    while (m) {
+                        const int l = __builtin_ctz(m);
+                        m &= m - 1;
+                        const uint32_t prim = g.prim[l];
+                        if (tri_mat[prim] != mat) continue;
+                        const Hit h{tf[l], prim, uf[l], vf[l]};
+                        if (out.cnt < 4) {
+                            out.rec[out.cnt++] = h;
+                        } else {
+                            int picked = -1;
+                            float t = 0.0f;
+                            for (int i = 0; i < 4; i++)
+                                if (t < out.rec[i].t) {
+                                    t = out.rec[i].t;
+                                    picked = i;
+                                }
+                            if (picked >= 0) out.rec[picked] = h;
+                            // ResolveMaxDepth: std::max(a, b) = a < b ? b : a
+                            float mx = out.rec[0].t < out.rec[1].t ? out.rec[1].t : out.rec[0].t;
+                            mx = mx < out.rec[2].t ? out.rec[2].t : mx;
+                            mx = mx < out.rec[3].t ? out.rec[3].t : mx;
+                            out.maxt = mx;
+                        }
+                    }
+                }
+                continue;
+            }
+            typename Simd<W>::T fm;
+            int m = IntersectBBox(ray, sr, node->bbox, fm);
+            if (0 == m) continue;
+            alignas(32) float f[W];
+            Simd<W>::store(f, fm);
+            const int k0 = __builtin_ctz(m);
+            const float t0 = f[k0];
+            m &= m - 1;
+            if (0 == m) {
+                stack[si++] = std::make_pair(node->children[k0].get(), t0);
+            } else {
+                const int k1 = __builtin_ctz(m);
+                m &= m - 1;
+                if (0 == m) {
+                    const float t1 = f[k1];
+                    if (t0 < t1) {
+                        stack[si++] = std::make_pair(node->children[k1].get(), t1);
+                        stack[si++] = std::make_pair(node->children[k0].get(), t0);
+                    } else {
+                        stack[si++] = std::make_pair(node->children[k0].get(), t0);
+                        stack[si++] = std::make_pair(node->children[k1].get(), t1);
+                    }
+                } else {
+                    for (unsigned i = 0; i < node->child_cnt; ++i) {
+                        int k = -1;
+                        float maxDist = -1.0f;
+                        for (unsigned j = 0; j < node->child_cnt; ++j) {
+                            if (f[j] > maxDist) {
+                                maxDist = f[j];
+                                k = (int)j;
+                            }
+                        }
+                        if (k == -1) break;
+                        f[k] = -1.0f;
+                        stack[si++] = std::make_pair(node->children[k].get(), maxDist);
+                    }
+                }
+            }
+        }
     }
 
     // Scene::GetAttenuation (core/scene.cpp:192-208) looping Accelerator::GetAttenuation
@@ -1036,6 +1193,36 @@ void oracle_attenuation(OracleHandle* h, const void* rays_, uint64_t n, const fl
             for (uint64_t i = lo; i < hi; i++) {
                 if (h->width == 8) B<8>(h)->Attenuation(toRay(rays[i]), transp, out + 4 * i, s8, max_steps);
                 else B<4>(h)->Attenuation(toRay(rays[i]), transp, out + 4 * i, s4, max_steps);
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nthreads; w++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+// BSSRDF multi-hit query of n rays: ray_mat[i] = the material ray i looks for, tri_mat = material
+// per triangle id; hits = 4 records per ray (t, prim, u, v; unused records FLT_MAX / 0xffffffff),
+// cnt = records filled.
+void oracle_multi(OracleHandle* h, const void* rays_, uint64_t n, const uint32_t* ray_mat, const uint32_t* tri_mat,
+                  void* hits_, uint32_t* cnt, int nthreads) {
+    const RayRec* rays = static_cast<const RayRec*>(rays_);
+    Hit* hits = static_cast<Hit*>(hits_);
+    std::atomic<uint64_t> next{0};
+    auto work = [&]() {
+        std::vector<std::pair<const Node<8>*, float>> s8;
+        std::vector<std::pair<const Node<4>*, float>> s4;
+        MultiHits mh;
+        while (true) {
+            const uint64_t lo = next.fetch_add(1024);
+            if (lo >= n) break;
+            const uint64_t hi = std::min(n, lo + 1024);
+            for (uint64_t i = lo; i < hi; i++) {
+                if (h->width == 8) B<8>(h)->GetIntersectMulti(toRay(rays[i]), tri_mat, ray_mat[i], mh, s8);
+                else B<4>(h)->GetIntersectMulti(toRay(rays[i]), tri_mat, ray_mat[i], mh, s4);
+                for (int k = 0; k < 4; k++) hits[4 * i + k] = mh.rec[k];
+                cnt[i] = mh.cnt;
             }
         }
     };

@@ -41,6 +41,7 @@ def lib():
         l.oracle_trace.argtypes = [_P, _P, ctypes.c_uint64, _P, _P, ctypes.c_int, ctypes.c_int, _P]
         l.oracle_brute.argtypes = [_P, _P, ctypes.c_uint64, _P, _P, ctypes.c_int]
         l.oracle_attenuation.argtypes = [_P, _P, ctypes.c_uint64, _P, _P, ctypes.c_uint32, ctypes.c_int]
+        l.oracle_multi.argtypes = [_P, _P, ctypes.c_uint64, _P, _P, _P, _P, ctypes.c_int]
         l.oracle_ray_setup.argtypes = [_P, _P, _P]
         l.oracle_tri_simd.restype = ctypes.c_int
         l.oracle_tri_simd.argtypes = [_P, _P, _P]
@@ -144,6 +145,17 @@ class OracleBvh:
         out = np.zeros((len(rays), 4), np.float32)
         lib().oracle_attenuation(self.h, _ptr(rays), len(rays), _ptr(tr), _ptr(out), max_steps, threads)
         return out
+
+    def multi(self, rays, ray_material, tri_material, threads=1):
+        """BSSRDF multi-hit query (Fbvh::GetIntersect with BSSRDFIntersections): ((n,4,4) records
+        [t, prim bits, u, v], (n,) counts)."""
+        rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 8))
+        rm = np.ascontiguousarray(np.asarray(ray_material, np.uint32).reshape(-1))
+        tm = np.ascontiguousarray(np.asarray(tri_material, np.uint32).reshape(-1))
+        hits = np.zeros((len(rays), 4, 4), np.float32)
+        cnt = np.zeros(len(rays), np.uint32)
+        lib().oracle_multi(self.h, _ptr(rays), len(rays), _ptr(rm), _ptr(tm), _ptr(hits), _ptr(cnt), threads)
+        return hits, cnt
 
     def brute_closest(self, rays):
         rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 8))

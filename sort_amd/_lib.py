@@ -119,6 +119,7 @@ _SIGS = {
     "sortgpu_trace_closest": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_any": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_attenuation": (_I, [_P, _P, _U32, _P, _U32, _P, _P]),
+    "sortgpu_trace_multi": (_I, [_P, _P, _U32, _P, _P, _U32, _P, _P, _P]),
     "sortgpu_trace_counters": (_I, [_P, _P, _U32, _I, _P, _P, _P, _P]),
     "sortgpu_ctx_set_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
     "sortgpu_pool_create": (_I, [_P, _P, _U32, ctypes.POINTER(_P)]),

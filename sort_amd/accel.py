@@ -191,6 +191,31 @@ class Accelerator:
                                               att.data_ptr(), s), self.ctx.handle)
         return att.cpu().numpy() if host else att
 
+    def multi(self, rays, ray_material, tri_material, stream=None):
+        """BSSRDF multi-hit query (SORT's GetIntersect with BSSRDFIntersections): ((n,4,4) float32
+        records [t, prim bits, u, v] in slot order, (n,) uint32 counts). ``ray_material``: (n,)
+        material per ray; ``tri_material``: material per triangle id."""
+        import torch
+        d_rays, host = self._device_rays(rays)
+        n = d_rays.shape[0]
+        dev = d_rays.device
+
+        def u32(x):
+            if isinstance(x, torch.Tensor):
+                assert x.is_cuda and x.dtype == torch.int32 and x.is_contiguous()
+                return x
+            return torch.from_numpy(np.ascontiguousarray(np.asarray(x, np.uint32).view(np.int32))).to(dev)
+
+        rm, tm = u32(ray_material), u32(tri_material)
+        hits = torch.empty((n, 4, 4), dtype=torch.float32, device=dev)
+        cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+        s = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        check(lib().sortgpu_trace_multi(self.handle, d_rays.data_ptr(), n, rm.data_ptr(), tm.data_ptr(), tm.numel(),
+                                        hits.data_ptr(), cnt.data_ptr(), s), self.ctx.handle)
+        if host:
+            return hits.cpu().numpy(), cnt.cpu().numpy().view(np.uint32)
+        return hits, cnt
+
     def counters(self, rays, mode: int = 0):
         """(results, {inner_nodes, leaves, tris, quads}) for closest (0) or any-hit (1)."""
         import torch

@@ -415,6 +415,45 @@ int sortgpu_trace_attenuation(sortgpu_accel* acc, const sortgpu_ray* d_rays, uin
     return SORTGPU_OK;
 }
 
+int sortgpu_trace_multi(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, const uint32_t* d_ray_material,
+                        const uint32_t* d_tri_material, uint32_t n_tri_material, sortgpu_hit* d_hits,
+                        uint32_t* d_counts, void* stream) {
+    if (!acc) {
+        set_error(nullptr, "trace_multi: accel is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    sortgpu_ctx* ctx = acc->ctx;
+    if (ctx->host_only) {
+        set_error(ctx, "trace_multi: host-only context has no device");
+        return SORTGPU_ERR_STATE;
+    }
+    if (n == 0) return SORTGPU_OK;
+    if (!d_rays || !d_ray_material || !d_hits || !d_counts || (!d_tri_material && !acc->idx.empty())) {
+        set_error(ctx, "trace_multi: null ray, material, hit or count pointer");
+        return SORTGPU_ERR_ARG;
+    }
+    const uint64_t ntris = acc->idx.size() / 3;
+    if (n_tri_material < ntris) {
+        set_error(ctx, "trace_multi: %u triangle materials for %llu triangles", n_tri_material,
+                  (unsigned long long)ntris);
+        return SORTGPU_ERR_ARG;
+    }
+    int rc = ctx_bind(ctx);
+    if (rc != SORTGPU_OK) return rc;
+    MultiArgs a{};
+    a.tree = acc->view;
+    a.rays = d_rays;
+    a.n = n;
+    a.ray_material = d_ray_material;
+    a.tri_material = d_tri_material;
+    a.hits = d_hits;
+    a.counts = d_counts;
+    a.error = acc->d_error;
+    hipError_t e = launch_multi(a, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "multi-hit launch");
+    return SORTGPU_OK;
+}
+
 int sortgpu_trace_counters(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, int mode,
                            sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4], void* stream) {
     if (!out || (mode != 0 && mode != 1)) {

@@ -44,6 +44,19 @@ hipError_t launch_attenuation(const TraceArgs& a, hipStream_t stream, int num_cu
 // Most surfaces one attenuation walk passes before it gives up (error bit 2).
 constexpr uint32_t kMaxAttenuationSteps = 4096;
 
+// BSSRDF multi-hit query (multihit.hip): per ray four (t, prim, u, v) records and a count.
+struct MultiArgs {
+    TreeView tree;
+    const void* rays;               // n x 32 B ray records (device)
+    uint32_t n;
+    const uint32_t* ray_material;   // per ray: the material its hits must have
+    const uint32_t* tri_material;   // per triangle id: its material
+    void* hits;                     // n x 4 x 16 B records
+    uint32_t* counts;               // n records filled
+    uint32_t* error;                // bit0: stack overflow
+};
+hipError_t launch_multi(const MultiArgs& a, hipStream_t stream);
+
 // Traversal order of a ray batch (order.hip): ray indices sorted by the origin's cell in the tree
 // box, so that a wave's lanes walk nearby nodes. Buffers come from `pool`, stream-ordered on `s`;
 // release_order frees them on `s` after the launch that reads `order`.
