@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tris", type=int, default=1 << 20)
-    p.add_argument("--legs", default="intersection,attenuation,pt,batched,served")
+    p.add_argument("--legs", default="intersection,attenuation,multi,pt,batched,served")
     p.add_argument("--served-requests", type=int, default=60)
     p.add_argument("--copy-probe", action="store_true",
                    help="end with three 256 MiB device copies (HBM counter calibration)")
@@ -210,6 +210,59 @@ def leg_attenuation(acc, d_sh, sh, pos, idx, stream, a, cpu=True):
     return rec
 
 
+def leg_multi(acc, inc, pos, idx, stream, a, cpu=True):
+    """BSSRDF probes (Fbvh::GetIntersect with BSSRDFIntersections): segments of length 0.04 through
+    the 4M incoherent rays' surface points along their directions (starting 0.02 before the point,
+    window (0, 0.04]: the root test rejects rays that start inside the scene box with a negative
+    tmin), four materials at random over the triangles, each probe looking for the material of the
+    surface its segment meets first (found with a closest-hit query before the timed region). GPU
+    rate over the whole batch; the oracle on 1M of the probes."""
+    import torch
+    from sort_amd._lib import check, lib as _l
+    probes = inc.copy()
+    probes[:, 0:3] = probes[:, 0:3] - np.float32(0.02) * probes[:, 4:7]
+    probes[:, 3] = np.float32(0.0)
+    probes[:, 7] = np.float32(0.04)
+    n = len(probes)
+    g = np.random.default_rng(2025)
+    tri_mat = g.integers(0, 4, len(idx)).astype(np.uint32)
+    first = acc.get_intersect(probes)[:, 1].copy().view(np.uint32)
+    ray_mat = np.where(first != 0xFFFFFFFF, tri_mat[np.minimum(first, len(idx) - 1)], 0).astype(np.uint32)
+    dev = torch.device("cuda:0")
+    d_p = torch.from_numpy(probes).to(dev)
+    d_rm = torch.from_numpy(ray_mat.view(np.int32)).to(dev)
+    d_tm = torch.from_numpy(tri_mat.view(np.int32)).to(dev)
+    hits = torch.empty((n, 4, 4), dtype=torch.float32, device=dev)
+    cnt = torch.empty((n,), dtype=torch.int32, device=dev)
+    lib = _l()
+
+    def run():
+        check(lib.sortgpu_trace_multi(acc.handle, d_p.data_ptr(), n, d_rm.data_ptr(), d_tm.data_ptr(), len(idx),
+                                      hits.data_ptr(), cnt.data_ptr(), stream.cuda_stream))
+
+    _, ms, _ = timed(run, max(5, a.steps // 5), 2, stream)
+    c = cnt.cpu().numpy()
+    rec = {"workload": f"{n} probe segments of length 0.04 through surface points, 4 materials",
+           "mrays_s": round(n / (ms * 1e-3) / 1e6, 3), "launch_ms_mean": round(ms, 4),
+           "records_per_ray": round(float(c.mean()), 4), "full_frac": round(float((c == 4).mean()), 4)}
+    if cpu:
+        from oracle.oracle import OracleBvh
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        ob = OracleBvh(pos, idx, width=a.width)
+        m = 1 << 20
+        ob.multi(probes[: 1 << 14], ray_mat[: 1 << 14], tri_mat, threads=threads)  # warm-up
+        t0 = time.perf_counter()
+        ref, ref_c = ob.multi(probes[:m], ray_mat[:m], tri_mat, threads=threads)
+        secs = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(m / secs / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} probes, one run after a warm-up"}
+        rec["gpu_over_cpu"] = round(rec["mrays_s"] / rec["cpu_baseline"]["value"], 2)
+        h = hits[:m].cpu().numpy()
+        rec["sample_bitwise_equal"] = bool(np.array_equal(ref.view(np.uint32), h.view(np.uint32)) and
+                                           np.array_equal(ref_c, c[:m].view(np.uint32)))
+    return rec
+
+
 def memory_pipeline(n, st, mean_ms, value, stream, width):
     """The ceiling the traversal runs against: each lane reads its own records (a node as 14
     consecutive 16-byte chunks, a triangle record as 3), so a launch is bound by the rate at which
@@ -324,6 +377,9 @@ def main():
     att = None
     if "attenuation" in set(a.legs.split(",")):
         att = leg_attenuation(acc, d_sh, sh, pos, idx, stream, a, cpu=not a.no_cpu)
+    multi = None
+    if "multi" in set(a.legs.split(",")):
+        multi = leg_multi(acc, inc, pos, idx, stream, a, cpu=not a.no_cpu)
 
     b_inc = bytes_per_ray(st_inc, 16)
     achieved = b_inc * n / (mean_ms * 1e-3) / 1e9
@@ -383,6 +439,8 @@ def main():
     out["roofline"]["memory_pipeline"] = memory_pipeline(n, st_inc, mean_ms, inc_mrays, stream, a.width)
     if att:
         out["attenuation"] = att
+    if multi:
+        out["multi"] = multi
     legs = set(a.legs.split(","))
     del d_inc, d_sh, d_cam, hits, occ
     acc.release()
