@@ -423,6 +423,44 @@ const sortgpu_scene_file_info* sortgpu_scene_file_get(const sortgpu_scene_file* 
 const char* sortgpu_scene_file_material_name(const sortgpu_scene_file* file, uint32_t i);
 void sortgpu_scene_file_destroy(sortgpu_scene_file* file);
 
+/* ---- film output (host + device; no counterpart needed beyond the file and socket formats) ----
+ * The film is width x height x 3 floats, row-major from the top row (the layout sortgpu_render
+ * writes), in device memory (on_device = 1, converted by a kernel on `stream`; the call
+ * synchronises it) or host memory (on_device = 0). */
+
+/* Bytes of the EXR file sortgpu_film_encode_exr writes for a width x height film. */
+uint64_t sortgpu_exr_size(uint32_t width, uint32_t height);
+/* The OpenEXR file SORT writes for a finished render (Texture2DBase::Output,
+ * texture/texturebase.cpp:27-57, through tinyexr's SaveEXR(rgb, w, h, 3, 1)), byte for byte:
+ * scanline, channels B G R in half floats (tinyexr's float_to_half_full rounding), no
+ * compression. Images below 16 x 16 are refused, as SaveEXR refuses them. out: host memory of at
+ * least sortgpu_exr_size bytes. */
+int sortgpu_film_encode_exr(const float* film_rgb, int on_device, uint32_t width, uint32_t height, uint8_t* out,
+                            uint64_t cap, void* stream);
+int sortgpu_film_write_exr(const float* film_rgb, int on_device, uint32_t width, uint32_t height, const char* path,
+                           void* stream);
+
+/* Display server packets (DisplayTile / DisplayImageInfo::Process, core/display_mgr.cpp:86-140):
+ * BLENDER = the add-on's tile socket (blender-plugin/addons/sortblend/renderer.py:31-86): [bytes
+ * after the length, w, h, x, y] + RGBA float32 from the tile's bottom row up; TEV = three
+ * UpdateImage packets (R, G, B) of the tile for the tev viewer's socket; TEV_CREATE = its
+ * CreateImage packet for the whole image (tile arguments ignored). The film is host memory. Returns
+ * the packet size (out = NULL sizes it) or a negative status. */
+#define SORTGPU_DISPLAY_BLENDER 0
+#define SORTGPU_DISPLAY_TEV 1
+#define SORTGPU_DISPLAY_TEV_CREATE 2
+int64_t sortgpu_display_packet(int mode, const char* title, const float* film_rgb, uint32_t width, uint32_t height,
+                               uint32_t x, uint32_t y, uint32_t tw, uint32_t th, uint8_t* out, uint64_t cap);
+
+typedef struct sortgpu_display sortgpu_display;
+/* TCP connection to a display server (DisplayManager::SetupDisplayServer, display_mgr.cpp:33-40). */
+int sortgpu_display_connect(const char* host, uint16_t port, sortgpu_display** out);
+/* Sends a film as tiles of `tile` pixels (0 = 64, the renderer's tile size) in row-major tile
+ * order: BLENDER or TEV mode (TEV sends CreateImage first). */
+int sortgpu_display_send_film(sortgpu_display* disp, int mode, const char* title, const float* film_rgb, int on_device,
+                              uint32_t width, uint32_t height, uint32_t tile, void* stream);
+void sortgpu_display_close(sortgpu_display* disp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -151,6 +151,14 @@ _SIGS = {
     "sortgpu_scene_file_get": (ctypes.POINTER(SceneFileInfo), [_P]),
     "sortgpu_scene_file_material_name": (ctypes.c_char_p, [_P, _U32]),
     "sortgpu_scene_file_destroy": (None, [_P]),
+    "sortgpu_exr_size": (ctypes.c_uint64, [_U32, _U32]),
+    "sortgpu_film_encode_exr": (_I, [_P, _I, _U32, _U32, _P, ctypes.c_uint64, _P]),
+    "sortgpu_film_write_exr": (_I, [_P, _I, _U32, _U32, ctypes.c_char_p, _P]),
+    "sortgpu_display_packet": (ctypes.c_int64, [_I, ctypes.c_char_p, _P, _U32, _U32, _U32, _U32, _U32, _U32, _P,
+                                                ctypes.c_uint64]),
+    "sortgpu_display_connect": (_I, [ctypes.c_char_p, ctypes.c_uint16, ctypes.POINTER(_P)]),
+    "sortgpu_display_send_film": (_I, [_P, _I, ctypes.c_char_p, _P, _I, _U32, _U32, _U32, _P]),
+    "sortgpu_display_close": (None, [_P]),
 }
 
 
