@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tris", type=int, default=1 << 20)
-    p.add_argument("--legs", default="intersection,attenuation,multi,pt,batched,served")
+    p.add_argument("--legs", default="intersection,attenuation,multi,film,pt,batched,served")
     p.add_argument("--served-requests", type=int, default=60)
     p.add_argument("--copy-probe", action="store_true",
                    help="end with three 256 MiB device copies (HBM counter calibration)")
@@ -263,6 +263,27 @@ def leg_multi(acc, inc, pos, idx, stream, a, cpu=True):
     return rec
 
 
+def leg_film(stream):
+    """SORT's EXR output of a 3840 x 2160 film: from HBM (half conversion on the GPU, 6 bytes per
+    pixel copied to the host) and from host memory (conversion on one host core)."""
+    import torch
+    from sort_amd import film
+    w, h = 3840, 2160
+    g = torch.Generator(device="cuda:0").manual_seed(3)
+    d = torch.rand((h, w, 3), generator=g, device="cuda:0", dtype=torch.float32)
+    host = d.cpu().numpy()
+    film.encode_exr(d, stream=stream.cuda_stream)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dev_bytes = film.encode_exr(d, stream=stream.cuda_stream)
+    t_dev = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    host_bytes = film.encode_exr(host)
+    t_host = time.perf_counter() - t0
+    return {"workload": f"EXR of a {w}x{h} film ({len(dev_bytes)} bytes)", "device_film_ms": round(t_dev * 1e3, 2),
+            "host_film_ms": round(t_host * 1e3, 2), "bytes_equal": dev_bytes == host_bytes}
+
+
 def memory_pipeline(n, st, mean_ms, value, stream, width):
     """The ceiling the traversal runs against: each lane reads its own records (a node as 14
     consecutive 16-byte chunks, a triangle record as 3), so a launch is bound by the rate at which
@@ -442,6 +463,8 @@ def main():
     if multi:
         out["multi"] = multi
     legs = set(a.legs.split(","))
+    if "film" in legs:
+        out["film"] = leg_film(stream)
     del d_inc, d_sh, d_cam, hits, occ
     acc.release()
     torch.cuda.empty_cache()

@@ -161,7 +161,7 @@ int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint
  * default 16); blocks_per_cu = cap on resident workgroups per CU (default: occupancy limit);
  * leaf_batch = lanes of a wave holding a leaf that end a node-walking phase (default 32, 1..64);
  * leaf_min = a leaf-testing phase ends when fewer lanes hold a leaf (default 1, 1..64);
- * reorder = traversal order of sortgpu_trace_closest / _any / _attenuation batches: 0 (default)
+ * reorder = traversal order of sortgpu_trace_closest / _any / _attenuation / _multi batches: 0 (default)
  * sorts batches of at least 262,144 rays by origin cell (64^3 grid over the scene box) before the
  * launch (stream-ordered device memory of 16 bytes per ray plus sort workspace, from a pool the
  * context owns; a batch sharing one origin, such as camera rays, keeps its order and pays only the

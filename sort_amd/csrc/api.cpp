@@ -449,7 +449,11 @@ int sortgpu_trace_multi(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     a.hits = d_hits;
     a.counts = d_counts;
     a.error = acc->d_error;
-    hipError_t e = launch_multi(a, static_cast<hipStream_t>(stream));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    RayOrder ro;
+    a.order = traversal_order(acc, d_rays, n, s, ro);
+    hipError_t e = launch_multi(a, s);
+    if (ro.order) release_order(ro, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "multi-hit launch");
     return SORTGPU_OK;
 }

@@ -17,8 +17,10 @@ constexpr int kMultiBlock = 256;
 
 template <int W, int STACK>
 __global__ __launch_bounds__(kMultiBlock) void k_multi(MultiArgs a) {
-    const uint64_t i = (uint64_t)blockIdx.x * kMultiBlock + threadIdx.x;
-    if (i >= a.n) return;
+    const uint64_t slot_k = (uint64_t)blockIdx.x * kMultiBlock + threadIdx.x;
+    if (slot_k >= a.n) return;
+    // the k-th probe of the traversal order (a permutation of 0..n-1), results at its own index
+    const uint64_t i = a.order ? a.order[slot_k] : slot_k;
     const NodeRec<W>* __restrict__ nodes = static_cast<const NodeRec<W>*>(a.tree.nodes);
     const PrimRec* __restrict__ prims = a.tree.prims;
     const float4* rp = reinterpret_cast<const float4*>(a.rays) + 2 * i;

@@ -54,6 +54,7 @@ struct MultiArgs {
     void* hits;                     // n x 4 x 16 B records
     uint32_t* counts;               // n records filled
     uint32_t* error;                // bit0: stack overflow
+    const uint32_t* order = nullptr; // optional traversal order (as TraceArgs::order)
 };
 hipError_t launch_multi(const MultiArgs& a, hipStream_t stream);
 

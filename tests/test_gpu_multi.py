@@ -87,3 +87,28 @@ def test_multi_edge_cases(gpu_ctx):
     h, c = acc.multi(d_rays, d_rm, d_tm, stream=s.cuda_stream)
     s.synchronize()
     assert int(c.sum()) == 0
+
+
+def test_multi_order_changes_no_result(gpu_ctx):
+    """Batches of 2^18 probes or more are walked in origin-cell order by default; never (1), always
+    (2) and the default (0) give the same bits, equal to the oracle's."""
+    from sort_amd import batch
+    pos, idx = _dense_soup(1 << 14, 81, size=0.05)
+    tri_mat = np.random.default_rng(82).integers(0, 2, len(idx)).astype(np.uint32)
+    acc = Obvh(gpu_ctx).build(pos, idx)
+    ob = OracleBvh(pos, idx, width=8)
+    rays = scenes.random_rays(300_000, 83, lo=-0.2, hi=1.2)
+    ray_mat = np.random.default_rng(84).integers(0, 2, len(rays)).astype(np.uint32)
+    res = {}
+    try:
+        for mode in (1, 2, 0):
+            batch.set_tuning(gpu_ctx, reorder=mode)
+            print(f"[multi order] mode {mode}", flush=True)
+            res[mode] = acc.multi(rays, ray_mat, tri_mat)
+    finally:
+        batch.set_tuning(gpu_ctx)
+    for mode in (2, 0):
+        assert np.array_equal(res[mode][0].view(np.uint32), res[1][0].view(np.uint32))
+        assert np.array_equal(res[mode][1], res[1][1])
+    ref, ref_cnt = ob.multi(rays, ray_mat, tri_mat, threads=16)
+    assert np.array_equal(res[0][0].view(np.uint32), ref.view(np.uint32)) and np.array_equal(res[0][1], ref_cnt)
