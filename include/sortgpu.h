@@ -227,6 +227,9 @@ typedef struct sortgpu_scene_desc {
     uint32_t nmaterials;
     const float* quad_radiance;    /* nquads x 3 emitted radiance */
     float env_radiance[3];         /* constant environment (AmbientLight); all 0 = none */
+    float clamp;                   /* > 0: each sample's radiance is clamped to [0, clamp] per channel before
+                                      it is averaged (the job's clamping, image_evaluation.cpp:222-226);
+                                      0 = none; negative or NaN is refused */
 } sortgpu_scene_desc;
 
 /* Pinhole perspective camera (PerspectiveCamera, camera/perspective.cpp:26-104). */

@@ -246,6 +246,7 @@ def _pt_sigs():
     l.oracle_pt_create.argtypes = [_P, _P, ctypes.c_uint32, _P, ctypes.c_uint32, _P, _P, _P, _P, ctypes.c_uint32,
                                    _P, ctypes.c_uint32, _P, _P, _P]
     l.oracle_pt_destroy.argtypes = [_P]
+    l.oracle_pt_set_clamp.argtypes = [_P, ctypes.c_float]
     l.oracle_pt_render.restype = ctypes.c_double
     l.oracle_pt_render.argtypes = [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _P, _P]
@@ -304,6 +305,8 @@ class OraclePT:
         self.h = l.oracle_pt_create(self.bvh.h, _ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(nrm), _ptr(tan),
                                     _ptr(tm), _ptr(mats), len(mats), _ptr(q34), len(q34), _ptr(qr), _ptr(env),
                                     _ptr(bb))
+        if scene.get("clamp", 0.0):
+            l.oracle_pt_set_clamp(self.h, float(scene["clamp"]))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -642,6 +642,7 @@ struct PtScene {
     uint32_t nlights = 0;
     float bbox[6];
     uint32_t ntris = 0;
+    float clamp = 0.0f;  // > 0: ImageEvaluation's per-sample clamp (image_evaluation.cpp:222-226)
 };
 
 struct RayRec {
@@ -861,6 +862,8 @@ extern "C" {
 
 // cam10: eye[3], target[3], up[3], fov_y. mats: nmats MatIn records. quad_rad: nquads x 3.
 // quads34: nquads x (mat[16], inv[16], sx, sy) (the same quads the BVH was built with).
+void oracle_pt_set_clamp(void* h, float clamp) { static_cast<PtScene*>(h)->clamp = clamp; }
+
 void* oracle_pt_create(OracleHandle* bvh, const float* pos, uint32_t nverts, const uint32_t* idx, uint32_t ntris,
                        const float* nrm, const float* tan, const uint32_t* tri_mat, const void* mats, uint32_t nmats,
                        const float* quads34, uint32_t nquads, const float* quad_rad, const float* env3,
@@ -966,7 +969,15 @@ static double render_impl(void* p, const float* cam10, uint32_t width, uint32_t 
                     const Vec dc = normalize(Vec(cx / xScale, cy / yScale, 1.0f));
                     const Vec d = right * dc.x + up2 * dc.y + fwd * dc.z;
                     RayDump dd{dump_rays, dump_hits, dump_valid, ((size_t)(pix - pix0) * spp + k) * max_depth};
-                    const Spectrum l = li(sc, eye, d, 0.0f, seed, pix, k, max_depth, e, s, dump_rays ? &dd : nullptr);
+                    Spectrum l = li(sc, eye, d, 0.0f, seed, pix, k, max_depth, e, s, dump_rays ? &dd : nullptr);
+                    if (sc.clamp > 0.0f) {  // li.Clamp(0, clamping): clamp() of math/utils.h:89-93
+                        auto cl = [&](float x) {
+                            if (x > sc.clamp) x = sc.clamp;
+                            if (x < 0.0f) x = 0.0f;
+                            return x;
+                        };
+                        l = Spectrum(cl(l.r), cl(l.g), cl(l.b));
+                    }
                     if (l.IsValid()) radiance = radiance + l;
                     else --valid;
                 }

@@ -55,7 +55,7 @@ class Material(ctypes.Structure):
 class SceneDesc(ctypes.Structure):
     _fields_ = [("normals", ctypes.c_void_p), ("tangents", ctypes.c_void_p), ("tri_material", ctypes.c_void_p),
                 ("materials", ctypes.c_void_p), ("nmaterials", ctypes.c_uint32), ("quad_radiance", ctypes.c_void_p),
-                ("env_radiance", ctypes.c_float * 3)]
+                ("env_radiance", ctypes.c_float * 3), ("clamp", ctypes.c_float)]
 
 
 class Camera(ctypes.Structure):

@@ -403,11 +403,20 @@ __global__ __launch_bounds__(kPtBlock) void k_finalize(const WaveDesc* __restric
     const ChunkDev& c = ct.c[k];
     const uint32_t pix = g - c.pix_base;
     float4* film = reqs[c.req].film;
+    const float cl = reqs[c.req].clamp;
     float4 acc = film[pix];
     for (uint32_t s = 0; s < c.nsamp; s++) {
         const uint32_t p = c.path_base + s * c.npix + pix;
         const float4 l = ps.L[p], a = ps.c0[p], b = ps.c1[p];
-        const float r = l.x + a.x + b.x, gg = l.y + a.y + b.y, bb = l.z + a.z + b.z;
+        float r = l.x + a.x + b.x, gg = l.y + a.y + b.y, bb = l.z + a.z + b.z;
+        if (cl > 0.0f) {  // Spectrum::Clamp(0, clamping): clamp() of math/utils.h:89-93, NaN kept
+            r = r > cl ? cl : r;
+            r = r < 0.0f ? 0.0f : r;
+            gg = gg > cl ? cl : gg;
+            gg = gg < 0.0f ? 0.0f : gg;
+            bb = bb > cl ? cl : bb;
+            bb = bb < 0.0f ? 0.0f : bb;
+        }
         if (isfinite(r) && isfinite(gg) && isfinite(bb)) {
             acc.x += r;
             acc.y += gg;

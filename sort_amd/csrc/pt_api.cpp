@@ -190,6 +190,7 @@ int render_impl(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, ui
     R.max_depth = max_depth;
     R.npix = npix;
     R.film = sc->film;
+    R.clamp = sc->host.clamp;
     hipError_t e = hipEventRecord(sc->ev0, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
     if ((e = pt_zero_f4(sc->film, npix, s)) != hipSuccess) return hip_fail(ctx, e, "zero film");

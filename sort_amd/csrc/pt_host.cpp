@@ -154,6 +154,10 @@ int prepare_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scen
         set_error(ctx, "scene: materials is NULL");
         return SORTGPU_ERR_ARG;
     }
+    if (!(desc->clamp >= 0.0f)) {
+        set_error(ctx, "scene: clamp must be >= 0 (0 = none)");
+        return SORTGPU_ERR_ARG;
+    }
     if (nquads && !desc->quad_radiance) {
         set_error(ctx, "scene: quad_radiance is NULL but the accelerator has quads");
         return SORTGPU_ERR_ARG;
@@ -264,6 +268,7 @@ int prepare_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scen
         h.nmats = desc->nmaterials;
         h.env_light = env_idx;
         h.env_L = v3(E[0], E[1], E[2]);
+        h.clamp = desc->clamp;
     } catch (const std::bad_alloc&) {
         set_error(ctx, "scene: host allocation failed");
         return SORTGPU_ERR_OOM;

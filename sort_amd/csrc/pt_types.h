@@ -23,6 +23,7 @@ struct SceneDev {
     uint32_t ntris, nquads, nlights, nmats;
     int32_t env_light;         // index of the environment light, or -1
     V3 env_L;
+    float clamp;               // > 0: per-sample radiance clamp
 };
 
 struct CamDev {
@@ -39,6 +40,7 @@ struct ReqDev {
     uint32_t npix;
     CamDev cam;
     float4* film;        // npix accumulators: summed rgb of valid samples, valid count
+    float clamp;         // > 0: each sample's radiance is clamped to [0, clamp] (its scene's)
 };
 
 // A wave's work: chunks of whole samples of requests. Paths [path_base, path_base + npix*nsamp)

@@ -240,6 +240,7 @@ class HipDevice : public Device {
             R.max_depth = r.p.max_depth;
             R.npix = r.npix;
             R.film = static_cast<float4*>(r.film);
+            R.clamp = scenes_[r.p.scene].im.dev.clamp;
             ct.c[k] = ChunkDev{(uint32_t)k, c.sample0, c.nsamp, r.npix, path_base, pix_base};
             path_base += r.npix * c.nsamp;
             pix_base += r.npix;

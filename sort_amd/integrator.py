@@ -76,8 +76,9 @@ def camera(eye, target, up=(0, 1, 0), fov_y_deg=40.0):
 
 
 def scene_desc(materials, tri_material=None, normals=None, tangents=None, quad_radiance=None,
-               env_radiance=(0.0, 0.0, 0.0)):
-    """A ``sortgpu_scene_desc`` and the arrays it points into (keep them alive while it is used)."""
+               env_radiance=(0.0, 0.0, 0.0), clamp=0.0):
+    """A ``sortgpu_scene_desc`` and the arrays it points into (keep them alive while it is used).
+    ``clamp`` > 0 clamps each sample's radiance to [0, clamp] (a scene file's ``clamping``)."""
     keep = []
     d = _lib.SceneDesc()
 
@@ -97,6 +98,7 @@ def scene_desc(materials, tri_material=None, normals=None, tangents=None, quad_r
     d.nmaterials = len(materials)
     d.quad_radiance = arr(quad_radiance, np.float32)
     d.env_radiance[:] = [float(x) for x in env_radiance]
+    d.clamp = float(clamp)
     return d, keep
 
 
@@ -104,10 +106,10 @@ class PathTracing:
     """Shading data + path tracer over a built accelerator (``sort_amd.accel.Accelerator``)."""
 
     def __init__(self, accel, materials, tri_material=None, normals=None, tangents=None, quad_radiance=None,
-                 env_radiance=(0.0, 0.0, 0.0), max_recursive_depth=6):
+                 env_radiance=(0.0, 0.0, 0.0), max_recursive_depth=6, clamp=0.0):
         self.accel = accel
         self.max_recursive_depth = max_recursive_depth
-        d, self._keep = scene_desc(materials, tri_material, normals, tangents, quad_radiance, env_radiance)
+        d, self._keep = scene_desc(materials, tri_material, normals, tangents, quad_radiance, env_radiance, clamp)
         h = ctypes.c_void_p()
         check(lib().sortgpu_scene_create(accel.handle, ctypes.byref(d), ctypes.byref(h)), accel.ctx.handle)
         self.handle = h

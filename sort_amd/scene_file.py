@@ -84,7 +84,7 @@ class SceneFile:
         return {
             "pos": pos, "idx": idx, "quads": quads, "normals": normals, "uv": uv, "tri_material": tri_material,
             "materials": materials, "material_names": names, "quad_radiance": quad_radiance,
-            "env": tuple(i.env_radiance), "ntris": int(i.ntris), "camera": camera,
+            "env": tuple(i.env_radiance), "ntris": int(i.ntris), "camera": camera, "clamp": float(i.clamping),
             "config": {"threads": i.threads, "spp": i.spp, "width": i.image_width, "height": i.image_height,
                        "clamping": i.clamping, "integrator": i.integrator,
                        "max_recursive_depth": i.max_recursive_depth, "max_bssrdf_bounces": i.max_bssrdf_bounces,

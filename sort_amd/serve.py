@@ -52,8 +52,8 @@ class Server:
         return check(rc, self.ctx.handle)
 
     def add_scene(self, accel, materials, tri_material=None, normals=None, tangents=None, quad_radiance=None,
-                  env_radiance=(0.0, 0.0, 0.0)) -> int:
-        d, keep = scene_desc(materials, tri_material, normals, tangents, quad_radiance, env_radiance)
+                  env_radiance=(0.0, 0.0, 0.0), clamp=0.0) -> int:
+        d, keep = scene_desc(materials, tri_material, normals, tangents, quad_radiance, env_radiance, clamp)
         sid = ctypes.c_uint32()
         self._chk(lib().sortgpu_server_add_scene(self.handle, accel.handle, ctypes.byref(d), ctypes.byref(sid)))
         del keep

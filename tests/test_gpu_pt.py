@@ -158,3 +158,45 @@ def test_c1_ray_dump_first_million_rays(gpu_ctx):
     assert prim_eq.mean() >= 0.995
     # bit-identical rays have bit-identical hits (the traversal is exact)
     assert np.array_equal(g_hits[same].view(np.uint32), o_hits[same].view(np.uint32))
+
+
+def test_per_sample_clamp_matches_oracle(gpu_ctx):
+    """The scene's clamp (ImageEvaluation's clamping) on the GPU: the same keys give the oracle's
+    clamped image; the light seen head-on is exactly the clamp; clamp 0 is the unclamped render;
+    a negative clamp is refused. A scene file's clamping reaches the render through the loader."""
+    from oracle import scene_file_oracle as SF
+    from sort_amd import SortGpuError, scene_file
+    sc = dict(scenes.cornell_sphere(), clamp=0.5)
+    acc = Obvh(gpu_ctx).build(sc["pos"], sc["idx"], sc["quads"])
+
+    def pt(clamp):
+        return I.PathTracing(acc, sc["materials"], tri_material=sc["tri_material"], normals=sc["normals"],
+                             quad_radiance=sc["quad_radiance"], env_radiance=sc["env"], max_recursive_depth=5,
+                             clamp=clamp)
+    print("[pt] clamp 0.5", flush=True)
+    g = pt(0.5).render(I.camera(**sc["camera"]), 64, 64, 8, seed=7).cpu().numpy().reshape(-1, 3)
+    o = OraclePT(sc).render(sc["camera"], 64, 64, 8, 5, 7, threads=16)
+    assert np.all(g <= np.float32(0.5)) and np.all(g >= 0)
+    close = np.abs(_lum(g) - _lum(o)) <= 1e-3 * np.maximum(_lum(o), 1e-3)
+    assert close.mean() > 0.97 and rel_mse(g, o) < 1e-4
+    up = dict(eye=(0.5, 0.9, 0.5), target=(0.5, 1.0, 0.5), up=(0.0, 0.0, 1.0), fov_y_deg=5.0)
+    head_on = pt(5.0).render(I.camera(**up), 16, 16, 4, seed=3).cpu().numpy()
+    assert np.all(head_on == np.float32(5.0))
+    a = pt(0.0).render(I.camera(**sc["camera"]), 32, 32, 4, seed=9).cpu().numpy()
+    b = I.PathTracing(acc, sc["materials"], tri_material=sc["tri_material"], normals=sc["normals"],
+                      quad_radiance=sc["quad_radiance"], env_radiance=sc["env"], max_recursive_depth=5).render(
+        I.camera(**sc["camera"]), 32, 32, 4, seed=9).cpu().numpy()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    with pytest.raises(SortGpuError, match="clamp"):
+        pt(-1.0)
+    # the loader's clamping reaches the render
+    data = bytearray(SF.export_scene_dict(scenes.cornell_sphere(), width=32, height=32, spp=4))
+    # clamping is the float after version, resource path, threads, spp, width and height
+    k = 4 + len(b"/tmp/sort/\0") + 16
+    data[k:k + 4] = np.float32(0.5).tobytes()
+    loaded = scene_file.parse(bytes(data))
+    assert loaded["clamp"] == np.float32(0.5)
+    g = I.PathTracing(acc, loaded["materials"], tri_material=loaded["tri_material"], normals=loaded["normals"],
+                      quad_radiance=loaded["quad_radiance"], env_radiance=loaded["env"], max_recursive_depth=5,
+                      clamp=loaded["clamp"]).render(I.camera(**loaded["camera"]), 32, 32, 4, seed=5).cpu().numpy()
+    assert np.all(g <= np.float32(0.5))

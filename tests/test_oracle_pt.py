@@ -252,3 +252,23 @@ def test_ray_dump_is_consistent_with_the_traversal_and_the_camera():
     m = v[:, :, 1]
     p = r0[..., 0:3] + h0[..., 0:1] * r0[..., 4:7]
     assert np.allclose(r1[m][:, 0:3], p[m], atol=1e-5) and np.all(r1[m][:, 3] == np.float32(1e-4))
+
+
+def test_per_sample_clamp_closed_forms():
+    """ImageEvaluation's clamping (image_evaluation.cpp:222-226): every sample's radiance is clamped
+    to [0, clamp] before the mean. Looking straight at the 17-radiance light from below, every
+    sample carries the light's emission (plus what its white surface reflects), so with a clamp of
+    5 every sample is 5 and so is the pixel; a clamp above every sample changes no bit; a clamped
+    image never exceeds the clamp."""
+    sc = scenes.cornell_sphere()
+    cam = dict(eye=(0.5, 0.9, 0.5), target=(0.5, 1.0, 0.5), up=(0.0, 0.0, 1.0), fov_y_deg=5.0)
+    plain = O.OraclePT(sc).render(cam, 8, 8, 4, 5, 3, threads=4)
+    assert np.all(plain >= np.float32(17.0))
+    clamped = O.OraclePT(dict(sc, clamp=5.0)).render(cam, 8, 8, 4, 5, 3, threads=4)
+    assert np.all(clamped == np.float32(5.0))
+    room = sc["camera"]
+    a = O.OraclePT(sc).render(room, 24, 24, 4, 5, 9, threads=8)
+    b = O.OraclePT(dict(sc, clamp=1e30)).render(room, 24, 24, 4, 5, 9, threads=8)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    c = O.OraclePT(dict(sc, clamp=0.5)).render(room, 24, 24, 4, 5, 9, threads=8)
+    assert np.all(c <= np.float32(0.5)) and np.all(c >= 0) and np.any(a > 0.5)
