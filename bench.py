@@ -285,37 +285,44 @@ def leg_film(stream):
 
 
 def memory_pipeline(n, st, mean_ms, value, stream, width):
-    """The ceiling the traversal runs against: each lane reads its own records (a node as 14
-    consecutive 16-byte chunks, a triangle record as 3), so a launch is bound by the rate at which
-    compute units return such chunks to lanes. The probe library replays those two access
-    patterns on an L2-resident buffer (the traversal's upper levels live in L2; its own L2 hit
-    rate is lower, so this is an upper bound) and gives chunks/s; the ceiling is the launch time
-    those rates allow for this launch's chunk counts."""
+    """The ceiling the traversal runs against: each lane reads its own records (an 8-wide node as
+    8 consecutive 16-byte chunks, 14 when it has more than four children; a 4-wide node as 7; a
+    triangle record as 3), so a launch is bound by the rate at which compute units return such
+    chunks to lanes. The probe library replays those access patterns on an L2-resident buffer
+    (the traversal's upper levels live in L2; its own L2 hit rate is lower, so this is an upper
+    bound) and gives chunks/s; the ceiling is the launch time those rates allow for this launch's
+    chunk counts."""
     import torch
 
     from sort_amd._lib import probe_lib
     p = probe_lib()
     node_chunks = 14 if width == 8 else 7
+    narrow_chunks = 8 if width == 8 else 7
     buf = torch.zeros(2 << 20, dtype=torch.uint8, device="cuda")  # 8,192 records of 256 B
     blocks = torch.cuda.get_device_properties(0).multi_processor_count * 12
     out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
     iters = 256
     rates = {}
-    for k in (node_chunks, 3):
+    for k in sorted({node_chunks, narrow_chunks, 3}):
         def run():
             rc = p.sortgpu_probe_chunks(buf.data_ptr(), 8192, k, iters, blocks, out.data_ptr(), stream.cuda_stream)
             if rc != 0:
                 raise RuntimeError(f"probe launch failed: {rc}")
         _, ms, _ = timed(run, 5, 2, stream)
         rates[k] = blocks * 256 * iters * k / (ms * 1e-3) / 1e9  # G chunks/s
-    per_ray = {"node": node_chunks * st["inner_nodes"] / n, "tri": 3 * (st["tris"] + st["quads"]) / n, "ray": 2.0}
-    ns_per_ray = per_ray["node"] / rates[node_chunks] + (per_ray["tri"] + per_ray["ray"]) / rates[3]
+    wide = st.get("wide_nodes", 0) if width == 8 else st["inner_nodes"]
+    narrow = st["inner_nodes"] - wide
+    per_ray = {"wide_node": node_chunks * wide / n, "node": narrow_chunks * narrow / n,
+               "tri": 3 * (st["tris"] + st["quads"]) / n, "ray": 2.0}
+    ns_per_ray = (per_ray["wide_node"] / rates[node_chunks] + per_ray["node"] / rates[narrow_chunks] +
+                  (per_ray["tri"] + per_ray["ray"]) / rates[3])
     ceiling = 1e3 / ns_per_ray  # Mrays/s
     chunks = sum(per_ray.values())
     return {"bound": "vector-memory return rate (16-byte chunks to lanes)",
             "chunks_per_ray": round(chunks, 1),
             "achieved_gchunks_s": round(chunks * n / (mean_ms * 1e-3) / 1e9, 1),
-            "probe_gchunks_s": {f"{node_chunks}_per_lane": round(rates[node_chunks], 1), "3_per_lane": round(rates[3], 1)},
+            "wide_node_frac": round(wide / max(st["inner_nodes"], 1), 4),
+            "probe_gchunks_s": {f"{k}_per_lane": round(v, 1) for k, v in sorted(rates.items(), reverse=True)},
             "ceiling_mrays_s": round(ceiling, 1), "frac": round(value / ceiling, 4)}
 
 

@@ -148,10 +148,11 @@ int sortgpu_trace_attenuation(sortgpu_accel* accel, const sortgpu_ray* d_rays, u
 
 /* Traversal work counters for n rays, synchronous: out[0..3] = inner nodes tested, leaves
  * visited, triangles tested, quads tested (the reference's OBVH statistics,
- * fast_bvh.hpp:74-97,407). mode 0 = closest, 1 = any-hit. Results are also written to
- * d_hits / d_occluded exactly as by the plain entry points. */
+ * fast_bvh.hpp:74-97,407); out[4] = node tests of 8-wide nodes with more than four children (the
+ * tests that load all 14 chunks of the record; the others load 8). mode 0 = closest, 1 = any-hit.
+ * Results are also written to d_hits / d_occluded exactly as by the plain entry points. */
 int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n, int mode,
-                           sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4],
+                           sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[5],
                            void* stream);
 
 /* ---- tuning (no counterpart in SORT: the knobs of the persistent traversal kernels) ---- */

@@ -12,7 +12,7 @@ extern "C" {
 #endif
 
 /* Launches `blocks` workgroups of 256 lanes on `stream` (a hipStream_t, NULL = default stream).
- * Every lane reads, `iters` times, `chunks` (1, 3, 7, 14 or 16) consecutive 16-byte chunks of a
+ * Every lane reads, `iters` times, `chunks` (1, 3, 7, 8, 14 or 16) consecutive 16-byte chunks of a
  * pseudo-random 256-byte record of `buf` (a device buffer of n_rec * 256 bytes), all chunks of a
  * record issued back to back, and writes one float to out[lane] (blocks * 256 floats). Returns 0
  * or a hipError_t code (other chunk counts are refused); asynchronous. */

@@ -217,11 +217,12 @@ class Accelerator:
         return hits, cnt
 
     def counters(self, rays, mode: int = 0):
-        """(results, {inner_nodes, leaves, tris, quads}) for closest (0) or any-hit (1)."""
+        """(results, {inner_nodes, leaves, tris, quads, wide_nodes}) for closest (0) or any-hit (1);
+        wide_nodes = tests of 8-wide nodes with more than four children."""
         import torch
         d_rays, host = self._device_rays(rays)
         n = d_rays.shape[0]
-        out = np.zeros(4, np.uint64)
+        out = np.zeros(5, np.uint64)
         s = torch.cuda.current_stream(d_rays.device).cuda_stream
         if mode == 0:
             res = torch.empty((n, 4), dtype=torch.float32, device=d_rays.device)
@@ -231,7 +232,7 @@ class Accelerator:
             res = torch.empty((n,), dtype=torch.uint8, device=d_rays.device)
             check(lib().sortgpu_trace_counters(self.handle, d_rays.data_ptr(), n, 1, None, res.data_ptr(),
                                                out.ctypes.data, s), self.ctx.handle)
-        st = dict(zip(["inner_nodes", "leaves", "tris", "quads"], [int(x) for x in out]))
+        st = dict(zip(["inner_nodes", "leaves", "tris", "quads", "wide_nodes"], [int(x) for x in out]))
         return (res.cpu().numpy() if host else res), st
 
 

@@ -312,7 +312,7 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
         return SORTGPU_ERR_STATE;
     }
     if (n == 0) {
-        if (out4) std::memset(out4, 0, 4 * sizeof(uint64_t));
+        if (out4) std::memset(out4, 0, 5 * sizeof(uint64_t));
         return SORTGPU_OK;
     }
     if (!d_rays || !d_out) {
@@ -334,7 +334,7 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     a.stats = nullptr;
     apply_tuning(ctx, a);
     if (stats) {
-        hipError_t e = hipMemsetAsync(acc->d_stats, 0, 4 * sizeof(unsigned long long), s);
+        hipError_t e = hipMemsetAsync(acc->d_stats, 0, 5 * sizeof(unsigned long long), s);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(stats)");
         a.stats = acc->d_stats;
     }
@@ -345,12 +345,12 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     if (ro.order) release_order(ro, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
     if (stats) {
-        unsigned long long h[4];
+        unsigned long long h[5];
         e = hipMemcpyAsync(h, acc->d_stats, sizeof(h), hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(stats)");
         e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
-        for (int k = 0; k < 4; k++) out4[k] = h[k];
+        for (int k = 0; k < 5; k++) out4[k] = h[k];
         uint32_t err = 0;
         e = hipMemcpy(&err, acc->d_error, sizeof(err), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(error)");
@@ -459,7 +459,7 @@ int sortgpu_trace_multi(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
 }
 
 int sortgpu_trace_counters(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, int mode,
-                           sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[4], void* stream) {
+                           sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[5], void* stream) {
     if (!out || (mode != 0 && mode != 1)) {
         set_error(acc ? acc->ctx : nullptr, "trace_counters: bad mode or null out");
         return SORTGPU_ERR_ARG;

@@ -58,6 +58,7 @@ extern "C" int sortgpu_probe_chunks(const void* buf, uint32_t n_rec, int chunks,
         case 1: return (int)launch<1>(buf, n_rec, iters, blocks, out, s);
         case 3: return (int)launch<3>(buf, n_rec, iters, blocks, out, s);
         case 7: return (int)launch<7>(buf, n_rec, iters, blocks, out, s);
+        case 8: return (int)launch<8>(buf, n_rec, iters, blocks, out, s);
         case 14: return (int)launch<14>(buf, n_rec, iters, blocks, out, s);
         case 16: return (int)launch<16>(buf, n_rec, iters, blocks, out, s);
         default: return (int)hipErrorInvalidValue;

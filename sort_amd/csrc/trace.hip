@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     int sp = 0;
     float bt = FLT_MAX, bu = 0.f, bv = 0.f;
     uint32_t bp = 0xffffffffu;
-    uint32_t n_nodes = 0, n_leaves = 0, n_tris = 0, n_quads = 0;
+    uint32_t n_nodes = 0, n_leaves = 0, n_tris = 0, n_quads = 0, n_wide = 0;
     bool overflow = false;
 
     auto push = [&](int e, uint32_t ref, float t) {
@@ -247,6 +247,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                         float f[W];
                         uint32_t ch[W];
                         const uint32_t m = node_test<W>(r, nodes, ref, f, ch);
+                        if (STATS && W == 8 && ch[4 % W] != REF_EMPTY) n_wide++;
                         const int nh = __popc(m);
                         uint32_t elig = m;
                         if (nh >= 3) {
@@ -364,9 +365,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 
     if (overflow) atomicOr(a.error, 1u);
     if (STATS) {
-        unsigned long long v[4] = {n_nodes, n_leaves, n_tris, n_quads};
+        unsigned long long v[5] = {n_nodes, n_leaves, n_tris, n_quads, n_wide};
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 5; k++) {
             unsigned long long x = v[k];
             for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
             if (lane == 0 && x) atomicAdd(a.stats + k, x);

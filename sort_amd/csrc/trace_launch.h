@@ -15,7 +15,7 @@ struct TraceArgs {
     void* out;                     // closest: n x 16 B hits; any: n x u8 (device)
     uint32_t* counter;             // work counter (device), reset by the launch
     uint32_t* error;               // error flags (device): bit0 stack overflow, bit1 bad scene id
-    unsigned long long* stats;     // 4 counters (device), only with stats
+    unsigned long long* stats;     // 5 counters (device), only with stats
     // multi-scene form: trees != null selects it
     const TreeView* trees;         // device table of tree views
     const uint32_t* ray_scene;     // per ray: index into trees (out-of-range ids miss, error bit1)

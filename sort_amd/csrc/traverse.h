@@ -193,23 +193,38 @@ __device__ __forceinline__ uint32_t node_test(const RayK& r, const NodeRec<W>* _
     const float4* base = reinterpret_cast<const float4*>(nodes + ni);
     float pl[6][W];
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            const float4 q = base[k * V + v];
-            pl[k][4 * v + 0] = q.x;
-            pl[k][4 * v + 1] = q.y;
-            pl[k][4 * v + 2] = q.z;
-            pl[k][4 * v + 3] = q.w;
-        }
-    }
-#pragma unroll
     for (int v = 0; v < V; v++) {
         const uint4 q = reinterpret_cast<const uint4*>(base)[6 * V + v];
         ch[4 * v + 0] = q.x;
         ch[4 * v + 1] = q.y;
         ch[4 * v + 2] = q.z;
         ch[4 * v + 3] = q.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const float4 q = base[k * V];
+        pl[k][0] = q.x;
+        pl[k][1] = q.y;
+        pl[k][2] = q.z;
+        pl[k][3] = q.w;
+    }
+    if (V == 2) {
+        // Children fill the slots in order, so slot 4 empty means slots 4..7 are: their planes
+        // are not needed (the REF_EMPTY test discards those slots) and are not loaded. Most
+        // 8-wide nodes have at most four children; the loads per lane are what bounds the walk.
+        if (ch[4 % W] != REF_EMPTY) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                const float4 q = base[k * V + (V - 1)];
+                pl[k][4 % W] = q.x;
+                pl[k][5 % W] = q.y;
+                pl[k][6 % W] = q.z;
+                pl[k][7 % W] = q.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; k++) pl[k][4 % W] = pl[k][5 % W] = pl[k][6 % W] = pl[k][7 % W] = 0.0f;
+        }
     }
     uint32_t mask = 0;
     if (r.ieee) {
