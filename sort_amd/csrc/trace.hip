@@ -25,7 +25,9 @@
 // with no closest-hit pruning before the first hit, occlusion does not depend on visit order.
 // Slab tests of rays with finite components against trees with bounded boxes use IEEE
 // min/max (equal to the x86 selects when no operand is NaN, which such rays cannot produce);
-// other rays take the select form (traverse.h, ieee_slabs_ok).
+// other rays take the select form (traverse.h, ieee_slabs_ok). An 8-wide node test loads the
+// second half of each plane only when the node has more than four children (traverse.h,
+// node_test): the 16-byte loads each lane issues are what bounds the walk.
 #include <hip/hip_runtime.h>
 
 #include "traverse.h"
