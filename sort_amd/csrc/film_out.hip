@@ -279,6 +279,11 @@ int64_t sortgpu_display_packet(int mode, const char* title, const float* film_rg
         set_error(nullptr, "display_packet: tile %u,%u %ux%u outside the %ux%u film", x, y, tw, th, width, height);
         return SORTGPU_ERR_ARG;
     }
+    // the packets' length fields are 32-bit signed integers
+    if (mode != SORTGPU_DISPLAY_TEV_CREATE && (uint64_t)tw * th * 16u + 4096u > 0x7fffffffull) {
+        set_error(nullptr, "display_packet: tile %ux%u too large for one packet", tw, th);
+        return SORTGPU_ERR_ARG;
+    }
     std::vector<uint8_t> v;
     try {
         if (mode == SORTGPU_DISPLAY_BLENDER) {
