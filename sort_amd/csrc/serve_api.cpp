@@ -44,7 +44,7 @@ int sortgpu_server_create(sortgpu_ctx* ctx, const sortgpu_server_opts* opts, sor
     eo.inflight = o.inflight_waves ? o.inflight_waves : 3;
     eo.manual = o.manual != 0;
     eo.fail_alloc_at = o.fail_alloc_at;
-    const uint32_t wave_paths = o.wave_paths ? o.wave_paths : (1u << 23);
+    const uint32_t wave_paths = o.wave_paths ? o.wave_paths : (1u << 25);
     if (eo.max_active > 32 || eo.inflight > 4 || wave_paths > (1u << 26)) {
         set_error(ctx, "sortgpu_server_create: max_active 1..32, inflight_waves 1..4, wave_paths <= 2^26");
         return SORTGPU_ERR_ARG;

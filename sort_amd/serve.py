@@ -303,7 +303,7 @@ def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, infligh
         res[name]["throughput_ratio_per_job"] = round(res[name]["one_at_a_time_per_job"]["wall_s"] / s["wall_s"], 3)
     for pt in pts:
         pt.close()
-    res["server"] = {"wave_paths": wave_paths or (1 << 23), "inflight_waves": inflight_waves or 3, "graphs": graphs}
+    res["server"] = {"wave_paths": wave_paths or (1 << 25), "inflight_waves": inflight_waves or 3, "graphs": graphs}
     res["requests"] = n_requests
     res["rate_per_s"] = rate
     res["paths"] = paths
