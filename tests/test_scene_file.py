@@ -269,3 +269,36 @@ def test_broken_streams_are_refused(tmp_path):
     # the oracle refuses the same streams
     with pytest.raises(ValueError):
         O.read_scene(data[: len(data) // 2])
+
+
+def test_corrupted_streams_parse_or_refuse():
+    # seeded corruption (bit flips, saturated bytes, NaN / huge words, truncation): every stream
+    # either parses or is refused with the library's error, never anything else
+    import random
+    rng = random.Random(20240611)
+    bases = [O.write_scene(**_random_scene(s)) for s in range(2)]
+    outcomes = {"parsed": 0, "refused": 0}
+    for _ in range(400):
+        b = bytearray(rng.choice(bases))
+        if rng.random() < 0.1:
+            b = b[:rng.randrange(len(b))]
+        for _ in range(rng.choice([1, 1, 2, 4])):
+            if not b:
+                break
+            k = rng.randrange(len(b))
+            mode = rng.random()
+            if mode < 0.5:
+                b[k] ^= 1 << rng.randrange(8)
+            elif mode < 0.8:
+                b[k] = rng.choice([0, 0xFF, 0x7F, 0x80])
+            else:
+                k4 = k - k % 4
+                b[k4:k4 + 4] = rng.choice([b"\xff\xff\xff\xff", b"\x00\x00\x00\x80", b"\xff\xff\xff\x7f",
+                                           b"\x00\x00\xc0\x7f"])
+        try:
+            scene_file.parse(bytes(b))
+            outcomes["parsed"] += 1
+        except SortGpuError as e:
+            assert e.code == -6, e
+            outcomes["refused"] += 1
+    assert outcomes["parsed"] > 50 and outcomes["refused"] > 50, outcomes
