@@ -21,6 +21,7 @@
 #include <sys/socket.h>
 #include <unistd.h>
 
+#include <cerrno>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -195,6 +196,7 @@ int film_lines(const float* film, int on_device, uint32_t w, uint32_t h, std::ve
 bool send_all(int fd, const uint8_t* p, size_t n) {
     while (n > 0) {
         const ssize_t k = send(fd, p, n, MSG_NOSIGNAL);
+        if (k < 0 && errno == EINTR) continue;  // interrupted before any byte went out
         if (k <= 0) return false;
         p += k;
         n -= (size_t)k;
@@ -376,6 +378,7 @@ int sortgpu_display_connect(const char* host, uint16_t port, sortgpu_display** o
     sortgpu_display* d = new (std::nothrow) sortgpu_display();
     if (!d) {
         close(fd);
+        set_error(nullptr, "display_connect: host allocation failed");
         return SORTGPU_ERR_OOM;
     }
     d->fd = fd;
