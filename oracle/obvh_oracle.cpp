@@ -228,7 +228,9 @@ template <> struct Simd<4> {  // simd_intel.h:34-173
     static T mad(T a, T b, T c) { return _mm_add_ps(_mm_mul_ps(a, b), c); }
     static T pick(T m, T a, T b) { return _mm_blendv_ps(b, a, m); }
     static T ceq(T a, T b) { return _mm_cmpeq_ps(a, b); }
-    static T cneq(T a, T b) { return _mm_cmp_ps(a, b, _CMP_NEQ_OQ); }
+    // simd_intel.h:137-138: the SSE wrapper's != is unordered (true for NaN), unlike the AVX
+    // wrapper's _CMP_NEQ_OQ; its one use (det != 0) never sees a NaN det.
+    static T cneq(T a, T b) { return _mm_cmpneq_ps(a, b); }
     static T cle(T a, T b) { return _mm_cmple_ps(a, b); }
     static T clt(T a, T b) { return _mm_cmplt_ps(a, b); }
     static T cge(T a, T b) { return _mm_cmpge_ps(a, b); }
