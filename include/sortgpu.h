@@ -139,8 +139,8 @@ int sortgpu_trace_multi(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_
  * a transparent closest hit multiplies the attenuation and the walk goes on from the hit point
  * (o + t d) with (0.001, tmax - t), until a miss or a black attenuation. d_atten[4i..4i+3] = the
  * attenuation r, g, b and the number of surfaces passed (as a float). A walk that passes 4,096
- * surfaces stops there with its attenuation so far and sets the accelerator's error flag
- * (reported by sortgpu_trace_counters). n_transparency = records in d_transparency; fewer than the
+ * surfaces stops there with its attenuation so far and raises SORTGPU_FLAG_ATTEN_CAP.
+ * n_transparency = records in d_transparency; fewer than the
  * accelerator's primitives (triangles + quads) is refused (SORTGPU_ERR_ARG). Stream-ordered. */
 int sortgpu_trace_attenuation(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
                               const float* d_transparency, uint32_t n_transparency, float* d_atten,
@@ -150,10 +150,25 @@ int sortgpu_trace_attenuation(sortgpu_accel* accel, const sortgpu_ray* d_rays, u
  * visited, triangles tested, quads tested (the reference's OBVH statistics,
  * fast_bvh.hpp:74-97,407); out[4] = node tests of 8-wide nodes with more than four children (the
  * tests that load all 14 chunks of the record; the others load 8). mode 0 = closest, 1 = any-hit.
- * Results are also written to d_hits / d_occluded exactly as by the plain entry points. */
+ * Results are also written to d_hits / d_occluded exactly as by the plain entry points. Takes the
+ * accelerator's error flags afterwards (SORTGPU_ERR_STATE when any is raised; out[] is filled). */
 int sortgpu_trace_counters(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n, int mode,
                            sortgpu_hit* d_hits, uint8_t* d_occluded, uint64_t out[5],
                            void* stream);
+
+/* ---- error flags (no counterpart in SORT, which asserts on the host instead) ----
+ * The kernels cannot return a status, so a fault they detect raises a flag on the accelerator
+ * (or pool) and the call that launched them still returns SORTGPU_OK. The flags stay raised until
+ * they are taken: sortgpu_accel_errors / sortgpu_pool_errors, sortgpu_trace_counters and
+ * sortgpu_render_stats take them (read and clear in one atomic step, after the work queued on
+ * their stream) and return SORTGPU_ERR_STATE with a message naming each flag when any was raised. */
+#define SORTGPU_FLAG_STACK_OVERFLOW 1u /* a traversal stack was full: those rays' results are incomplete */
+#define SORTGPU_FLAG_BAD_SCENE 2u      /* sortgpu_pool_trace: a ray named a scene outside the pool (it missed) */
+#define SORTGPU_FLAG_ATTEN_CAP 4u      /* sortgpu_trace_attenuation: a walk passed 4,096 surfaces and stopped */
+
+/* Takes the flags raised by the accelerator's queries and renders since the last check, after the
+ * work queued on `stream` (synchronises it). *flags (may be NULL) receives them. */
+int sortgpu_accel_errors(sortgpu_accel* accel, void* stream, uint32_t* flags);
 
 /* ---- tuning (no counterpart in SORT: the knobs of the persistent traversal kernels) ---- */
 
@@ -197,6 +212,8 @@ int sortgpu_pool_info(const sortgpu_pool* pool, uint64_t out[4]);
  * the pool. any = 0: d_out is n sortgpu_hit; any = 1: n uint8 occlusion flags. */
 int sortgpu_pool_trace(sortgpu_pool* pool, const sortgpu_ray* d_rays, const uint32_t* d_ray_scene, uint32_t n,
                        int any, void* d_out, void* stream);
+/* Takes the flags raised by the pool's traces since the last check (as sortgpu_accel_errors). */
+int sortgpu_pool_errors(sortgpu_pool* pool, void* stream, uint32_t* flags);
 
 /* ---- path tracing (PathTracing::Li, integrator/pathtracing.cpp:37-258) ---- */
 
@@ -270,7 +287,8 @@ int sortgpu_li(sortgpu_scene* scene, const sortgpu_ray* d_rays, const uint32_t* 
                uint32_t max_depth, uint64_t seed, float* d_rgb, void* stream);
 
 /* Totals of the last render / li call (synchronous): out[0] extension rays, out[1] shadow
- * rays, out[2] paths, out[3] microseconds of the last call on its stream (event timing). */
+ * rays, out[2] paths, out[3] microseconds of the last call on its stream (event timing). Takes the
+ * accelerator's error flags afterwards (SORTGPU_ERR_STATE when any is raised; out[] is filled). */
 int sortgpu_render_stats(sortgpu_scene* scene, uint64_t out[4]);
 
 /* ---- Served form: continuous-batching render server ------------------------------------------

@@ -21,6 +21,11 @@ ERR_OOM = -3
 ERR_BUDGET = -4
 ERR_STATE = -5
 
+# error flags the kernels raise (taken by Obvh.check_errors / Pool.check_errors)
+FLAG_STACK_OVERFLOW = 1
+FLAG_BAD_SCENE = 2
+FLAG_ATTEN_CAP = 4
+
 
 class SortGpuError(RuntimeError):
     def __init__(self, code: int, msg: str):
@@ -121,6 +126,8 @@ _SIGS = {
     "sortgpu_trace_attenuation": (_I, [_P, _P, _U32, _P, _U32, _P, _P]),
     "sortgpu_trace_multi": (_I, [_P, _P, _U32, _P, _P, _U32, _P, _P, _P]),
     "sortgpu_trace_counters": (_I, [_P, _P, _U32, _I, _P, _P, _P, _P]),
+    "sortgpu_accel_errors": (_I, [_P, _P, ctypes.POINTER(_U32)]),
+    "sortgpu_pool_errors": (_I, [_P, _P, ctypes.POINTER(_U32)]),
     "sortgpu_ctx_set_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
     "sortgpu_pool_create": (_I, [_P, _P, _U32, ctypes.POINTER(_P)]),
     "sortgpu_pool_destroy": (None, [_P]),

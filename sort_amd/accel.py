@@ -236,6 +236,17 @@ class Accelerator:
         return (res.cpu().numpy() if host else res), st
 
 
+    def check_errors(self, stream=None) -> int:
+        """Takes the error flags the accelerator's kernels raised since the last check (after the
+        work queued on `stream`); raises SortGpuError naming each flag when any was raised, else
+        returns 0."""
+        import torch
+        flags = ctypes.c_uint32(0)
+        s = stream if stream is not None else torch.cuda.current_stream(self.ctx.device).cuda_stream
+        check(lib().sortgpu_accel_errors(self.handle, s, ctypes.byref(flags)), self.ctx.handle)
+        return flags.value
+
+
 class Obvh(Accelerator):
     """8-wide SAH BVH (SORT's Obvh, accel/obvh.h)."""
     width = 8

@@ -44,6 +44,16 @@ class Pool:
                                        out.data_ptr(), s), self.ctx.handle)
         return out
 
+    def check_errors(self, stream=None) -> int:
+        """Takes the error flags the pool's traces raised since the last check; raises
+        SortGpuError naming each flag when any was raised (e.g. a ray naming a scene outside the
+        pool), else returns 0."""
+        import torch
+        flags = ctypes.c_uint32(0)
+        s = stream if stream is not None else torch.cuda.current_stream(self.ctx.device).cuda_stream
+        check(lib().sortgpu_pool_errors(self.handle, s, ctypes.byref(flags)), self.ctx.handle)
+        return flags.value
+
     def close(self):
         if self.handle:
             lib().sortgpu_pool_destroy(self.handle)

@@ -68,6 +68,27 @@ int ctx_bind(sortgpu_ctx* ctx) {
     return SORTGPU_OK;
 }
 
+int report_flags(sortgpu_ctx* ctx, uint32_t* d_flags, hipStream_t s, const char* who, uint32_t* flags_out) {
+    // one taker at a time: the staging word d_flags[1] is shared by every check of the object
+    static std::mutex take_mu;
+    uint32_t f = 0;
+    hipError_t e;
+    {
+        std::lock_guard<std::mutex> l(take_mu);
+        e = take_flags(d_flags, s, &f);
+    }
+    if (flags_out) *flags_out = f;
+    if (e != hipSuccess) return hip_fail(ctx, e, "error flags");
+    if (!f) return SORTGPU_OK;
+    std::string msg;
+    if (f & kFlagStackOverflow) msg += "; traversal stack overflow (those rays' results are incomplete)";
+    if (f & kFlagBadScene) msg += "; a ray named a scene outside the pool (it missed)";
+    if (f & kFlagAttenCap) msg += "; an attenuation walk passed 4096 surfaces and stopped there";
+    if (f & ~(kFlagStackOverflow | kFlagBadScene | kFlagAttenCap)) msg += "; unknown flag";
+    set_error(ctx, "%s: flags %u%s", who, f, msg.c_str());
+    return SORTGPU_ERR_STATE;
+}
+
 }  // namespace sortgpu
 
 extern "C" {
@@ -351,13 +372,7 @@ static int trace_common(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
         e = hipStreamSynchronize(s);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
         for (int k = 0; k < 5; k++) out4[k] = h[k];
-        uint32_t err = 0;
-        e = hipMemcpy(&err, acc->d_error, sizeof(err), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(error)");
-        if (err) {
-            set_error(ctx, "trace: traversal stack overflow (flags %u)", err);
-            return SORTGPU_ERR_STATE;
-        }
+        return report_flags(ctx, acc->d_error, s, "trace_counters", nullptr);
     }
     return SORTGPU_OK;
 }
@@ -456,6 +471,18 @@ int sortgpu_trace_multi(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t 
     if (ro.order) release_order(ro, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "multi-hit launch");
     return SORTGPU_OK;
+}
+
+int sortgpu_accel_errors(sortgpu_accel* acc, void* stream, uint32_t* flags) {
+    if (flags) *flags = 0;
+    if (!acc) {
+        set_error(nullptr, "sortgpu_accel_errors: accel is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    if (acc->ctx->host_only) return SORTGPU_OK;
+    int rc = ctx_bind(acc->ctx);
+    if (rc != SORTGPU_OK) return rc;
+    return report_flags(acc->ctx, acc->d_error, static_cast<hipStream_t>(stream), "accelerator", flags);
 }
 
 int sortgpu_trace_counters(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, int mode,

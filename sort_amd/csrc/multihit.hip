@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kMultiBlock) void k_multi(MultiArgs a) {
             if (elig & (1u << j)) st[sp + (int)rank[j]] = make_uint2(ch[j], __float_as_uint(f[j]));
         sp += n;
     }
-    if (overflow) atomicOr(a.error, 1u);
+    if (overflow) atomicOr(a.error, kFlagStackOverflow);
     float4* out = reinterpret_cast<float4*>(a.hits) + 4 * i;
     out[0] = make_float4(rt0, __uint_as_float(rp0), ru0, rv0);
     out[1] = make_float4(rt1, __uint_as_float(rp1), ru1, rv1);

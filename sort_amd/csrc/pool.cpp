@@ -144,6 +144,17 @@ int sortgpu_pool_info(const sortgpu_pool* p, uint64_t out[4]) {
     return SORTGPU_OK;
 }
 
+int sortgpu_pool_errors(sortgpu_pool* p, void* stream, uint32_t* flags) {
+    if (flags) *flags = 0;
+    if (!p) {
+        set_error(nullptr, "sortgpu_pool_errors: pool is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    int rc = ctx_bind(p->ctx);
+    if (rc) return rc;
+    return report_flags(p->ctx, p->d_error, static_cast<hipStream_t>(stream), "pool", flags);
+}
+
 int sortgpu_pool_trace(sortgpu_pool* p, const sortgpu_ray* d_rays, const uint32_t* d_ray_scene, uint32_t n, int any,
                        void* d_out, void* stream) {
     if (!p) {

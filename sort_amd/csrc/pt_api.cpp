@@ -348,18 +348,11 @@ int sortgpu_render_stats(sortgpu_scene* sc, uint64_t out[4]) {
     float ms = 0.0f;
     e = hipEventElapsedTime(&ms, sc->ev0, sc->ev1);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventElapsedTime");
-    uint32_t err = 0;
-    e = hipMemcpy(&err, sc->accel->d_error, sizeof(err), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(error)");
     out[0] = t[0];
     out[1] = t[1];
     out[2] = sc->last_paths;
     out[3] = (uint64_t)(ms * 1000.0f);
-    if (err) {
-        set_error(ctx, "render: traversal stack overflow");
-        return SORTGPU_ERR_STATE;
-    }
-    return SORTGPU_OK;
+    return report_flags(ctx, sc->accel->d_error, nullptr, "render", nullptr);
 }
 
 }  // extern "C"

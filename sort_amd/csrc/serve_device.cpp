@@ -10,6 +10,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 
 #include "pt_host.h"
 
@@ -280,26 +281,43 @@ class HipDevice : public Device {
         }
         // the end event is recorded whatever happened, so the wave retires in order
         hipError_t e2 = hipEventRecord(ev.b, s);
-        evs_.push_back(ev);
+        {
+            std::lock_guard<std::mutex> l(ev_mu_);
+            evs_.push_back(ev);
+        }
         if (e != hipSuccess) return (err = hip_msg("wave launch", e), hip_rc(e));
         if (rc) return (err = sortgpu_last_error(ctx_), rc);
         if (e2 != hipSuccess) return (err = hip_msg("hipEventRecord", e2), hip_rc(e2));
         return SORTGPU_OK;
     }
 
+    // The blocking form runs without the engine's lock while other threads may launch waves: the
+    // event list is read and changed under ev_mu_ only, and the events of wave `seq` stay out of
+    // the spare list (so no launch can record them again) until this call retires it.
     int wait(uint64_t seq, bool block, bool* done, std::string& err) override {
         ctx_bind(ctx_);
         *done = false;
-        if (evs_.empty() || evs_.front().seq != seq) return (err = "wave bookkeeping out of order", SORTGPU_ERR_STATE);
-        Ev ev = evs_.front();
+        Ev ev;
+        {
+            std::lock_guard<std::mutex> l(ev_mu_);
+            if (evs_.empty() || evs_.front().seq != seq)
+                return (err = "wave bookkeeping out of order", SORTGPU_ERR_STATE);
+            ev = evs_.front();
+        }
         hipError_t e = block ? hipEventSynchronize(ev.b) : hipEventQuery(ev.b);
         if (e == hipErrorNotReady) return SORTGPU_OK;
         if (e != hipSuccess) return (err = hip_msg("wave", e), hip_rc(e));
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) wave_us_ += (uint64_t)(ms * 1000.0f);
-        evs_.pop_front();
-        spare_.push_back(ev.a);
-        spare_.push_back(ev.b);
+        const bool timed = hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess;
+        {
+            std::lock_guard<std::mutex> l(ev_mu_);
+            if (evs_.empty() || evs_.front().seq != seq)
+                return (err = "wave bookkeeping out of order", SORTGPU_ERR_STATE);
+            if (timed) wave_us_ += (uint64_t)(ms * 1000.0f);
+            evs_.pop_front();
+            spare_.push_back(ev.a);
+            spare_.push_back(ev.b);
+        }
         uint32_t flags = 0;
         if ((e = hipMemcpy(&flags, d_error_, sizeof(flags), hipMemcpyDeviceToHost)) != hipSuccess)
             return (err = hip_msg("error flags", e), hip_rc(e));
@@ -326,6 +344,7 @@ class HipDevice : public Device {
             out[0] += t[0];
             out[1] += t[1];
         }
+        std::lock_guard<std::mutex> l(ev_mu_);
         out[2] = wave_us_;
     }
 
@@ -413,6 +432,7 @@ class HipDevice : public Device {
         }
     }
     hipEvent_t take_event() {
+        std::lock_guard<std::mutex> l(ev_mu_);
         if (!spare_.empty()) {
             hipEvent_t e = spare_.back();
             spare_.pop_back();
@@ -452,6 +472,7 @@ class HipDevice : public Device {
     uint32_t slot_ = 0;
     std::vector<SceneHost> scenes_;
     uint32_t width_ = 0, max_stack_ = 1;
+    std::mutex ev_mu_;  // guards evs_, spare_ and wave_us_ (the blocking wait runs unlocked)
     std::deque<Ev> evs_;
     std::vector<hipEvent_t> spare_;
     std::multimap<size_t, void*> cache_;
@@ -514,11 +535,13 @@ class HostDevice : public Device {
                     a[3] += 1.0f;
                 }
         }
+        std::lock_guard<std::mutex> l(mu_);
         ext_ += w.paths;
         pending_.push_back({w.seq, polls_});
         return SORTGPU_OK;
     }
     int wait(uint64_t seq, bool block, bool* done, std::string& err) override {
+        std::lock_guard<std::mutex> l(mu_);
         *done = false;
         if (pending_.empty() || pending_.front().first != seq)
             return (err = "wave bookkeeping out of order", SORTGPU_ERR_STATE);
@@ -535,6 +558,7 @@ class HostDevice : public Device {
         return SORTGPU_OK;
     }
     void counters(uint64_t out[3]) override {
+        std::lock_guard<std::mutex> l(mu_);
         out[0] = ext_;
         out[1] = 0;
         out[2] = 0;
@@ -544,6 +568,7 @@ class HostDevice : public Device {
     sortgpu_ctx* ctx_;
     uint32_t cap_, polls_;
     std::vector<std::vector<float>*> films_;
+    std::mutex mu_;  // guards pending_ and ext_ (the engine's blocking wait runs unlocked)
     std::deque<std::pair<uint64_t, uint32_t>> pending_;
     uint64_t ext_ = 0;
 };

@@ -423,7 +423,14 @@ void Engine::retire(Wave& w) {
 }
 
 // Retires finished waves in order; with block, waits for the oldest one (mu released meanwhile).
+// One thread at a time may wait on the device: while it does (polling_), every other poll returns
+// at once, or with block waits until that thread has retired what finished, so the device's wait
+// is never entered twice and no wave is retired by two threads.
 void Engine::poll(std::unique_lock<std::mutex>& lk, bool block) {
+    if (polling_) {
+        if (block) cv_done_.wait(lk, [this] { return !polling_; });
+        return;
+    }
     bool first = true;
     while (!inflight_.empty()) {
         const uint64_t seq = inflight_.front().seq;
@@ -431,9 +438,14 @@ void Engine::poll(std::unique_lock<std::mutex>& lk, bool block) {
         std::string err;
         int rc;
         if (block && first) {
+            polling_ = true;
             lk.unlock();
             rc = dev_->wait(seq, true, &done, err);
             lk.lock();
+            polling_ = false;
+            cv_done_.notify_all();
+            // nothing else retires waves while polling_ is set; the check keeps that an invariant
+            if (!rc && (inflight_.empty() || inflight_.front().seq != seq)) break;
         } else {
             rc = dev_->wait(seq, false, &done, err);
         }

@@ -57,7 +57,8 @@ struct Wave {
     std::vector<Chunk> chunks;
 };
 
-// Device side of the engine. Only the engine's driving thread calls it.
+// Device side of the engine. The engine calls it with its lock held, except for one blocking
+// wait at a time, which runs unlocked while other threads may call the rest.
 class Device {
   public:
     virtual ~Device() {}
@@ -136,6 +137,7 @@ class Engine {
     std::condition_variable cv_work_, cv_done_;
     std::thread worker_;
     bool stop_ = false, started_ = false, broken_ = false;
+    bool polling_ = false;  // a thread is blocked in dev_->wait with mu_ released
     std::string broken_msg_;
     std::map<uint64_t, std::unique_ptr<Request>> reqs_;
     std::deque<Request*> queue_;    // QUEUED, arrival order

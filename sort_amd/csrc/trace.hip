@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                         } else {
                             root = 0;
                             valid = 0;
-                            atomicOr(a.error, 2u);
+                            atomicOr(a.error, kFlagBadScene);
                         }
                     } else {
                         root = a.tree.root;
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                     steps++;
                     if (!(att_r == 0.0f && att_g == 0.0f && att_b == 0.0f)) {
                         if (steps >= kMaxAttenuationSteps) {
-                            atomicOr(a.error, 4u);
+                            atomicOr(a.error, kFlagAttenCap);
                         } else {
                             r.ox = r.ox + bt * r.dx;  // ray(t) = o + t * d
                             r.oy = r.oy + bt * r.dy;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         occluded = false;
     }
 
-    if (overflow) atomicOr(a.error, 1u);
+    if (overflow) atomicOr(a.error, kFlagStackOverflow);
     if (STATS) {
         unsigned long long v[5] = {n_nodes, n_leaves, n_tris, n_quads, n_wide};
 #pragma unroll
@@ -379,6 +379,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 
 __global__ void k_reset(uint32_t* counter) {
     if (threadIdx.x == 0) *counter = 0;
+}
+
+// Moves the error flags into flags[1] and clears them in one atomic step, so a flag raised by a
+// kernel running meanwhile on another stream is either taken now or left for the next check.
+__global__ void k_take_flags(uint32_t* flags) {
+    if (threadIdx.x == 0) flags[1] = atomicExch(flags, 0u);
+}
+
+hipError_t take_flags(uint32_t* d_flags, hipStream_t stream, uint32_t* out) {
+    *out = 0;
+    hipLaunchKernelGGL(k_take_flags, dim3(1), dim3(64), 0, stream, d_flags);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
 }
 
 template <int W, bool ANY, bool STATS, int SPILL, bool MULTI, int LDS, bool TRANSP = false>

@@ -14,7 +14,7 @@ struct TraceArgs {
     const uint32_t* n_dev;         // optional device-side ray count (read at kernel start)
     void* out;                     // closest: n x 16 B hits; any: n x u8 (device)
     uint32_t* counter;             // work counter (device), reset by the launch
-    uint32_t* error;               // error flags (device): bit0 stack overflow, bit1 bad scene id
+    uint32_t* error;               // error flags (device): kFlag* bits
     unsigned long long* stats;     // 5 counters (device), only with stats
     // multi-scene form: trees != null selects it
     const TreeView* trees;         // device table of tree views
@@ -70,6 +70,13 @@ struct RayOrder {
 hipError_t order_rays(hipMemPool_t pool, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
                       RayOrder& ro);
 void release_order(RayOrder& ro, hipStream_t s);
+// Error flags the traversal kernels raise (atomicOr into a 16-byte device word block).
+constexpr uint32_t kFlagStackOverflow = 1u;  // a push found the stack full: that ray's result is incomplete
+constexpr uint32_t kFlagBadScene = 2u;       // multi-scene form: a ray named a tree outside the table
+constexpr uint32_t kFlagAttenCap = 4u;       // an attenuation walk passed kMaxAttenuationSteps surfaces
+// Takes the flags word d_flags[0] (clearing it) after the work queued on `stream`; d_flags[1]
+// is the staging word. Synchronises `stream`.
+hipError_t take_flags(uint32_t* d_flags, hipStream_t stream, uint32_t* out);
 // Deepest stack (entries) the kernels can hold.
 uint32_t trace_max_stack_supported();
 

@@ -2,7 +2,7 @@
 // tests/test_serve_sanitizers.py under ThreadSanitizer and under AddressSanitizer + UBSan.
 //
 // Several client threads submit, wait, cancel, release and read results while the engine's worker
-// admits, plans, launches and retires waves; the device completes waves on a thread of its own
+// (or, in manual mode, the clients' own wait calls) admits, plans, launches and retires waves; the device completes waves on a thread of its own
 // after a short delay and fails a scripted fraction of film allocations. Every finished image is
 // checked against the value the device's per-path hash gives in sample order, so a scheduling
 // race that reorders or drops samples fails the run even when no sanitizer fires.
@@ -168,11 +168,15 @@ bool check_image(const std::vector<float>& rgb, uint64_t seed, uint32_t scene, u
 
 }  // namespace
 
-int main() {
+// One run of six clients against one engine. manual: no worker thread; the clients' own wait and
+// drain calls drive the engine, so several threads poll it at once and take turns waiting on the
+// device with the engine's lock released.
+int run(bool manual) {
     Options o;
     o.budget = (1 << 20) + 4 * 300000 + 64 * 64 * 16 * 6;  // workspace + 4 scenes + six 64^2 films
     o.max_active = 6;
     o.inflight = 3;
+    o.manual = manual;
     Engine eng(std::unique_ptr<Device>(new ThreadDevice(4096)), o);
     std::string err;
     if (eng.start(err)) {
@@ -230,9 +234,16 @@ int main() {
     if (eng.drain(30.0)) bad++;
     uint64_t st[16];
     eng.stats(st);
-    std::printf("done %d failed %d cancelled %d bad %d waves %llu evictions %llu peak %llu budget %llu\n", done.load(),
-                failed.load(), cancelled.load(), bad.load(), (unsigned long long)st[0], (unsigned long long)st[5],
-                (unsigned long long)st[8], (unsigned long long)st[14]);
+    std::printf("%s: done %d failed %d cancelled %d bad %d waves %llu evictions %llu peak %llu budget %llu\n",
+                manual ? "manual" : "worker", done.load(), failed.load(), cancelled.load(), bad.load(),
+                (unsigned long long)st[0], (unsigned long long)st[5], (unsigned long long)st[8],
+                (unsigned long long)st[14]);
     if (st[8] > st[14]) bad++;
     return bad.load() == 0 && done.load() > 100 ? 0 : 1;
+}
+
+int main() {
+    const int a = run(false);
+    const int b = run(true);
+    return a | b;
 }
