@@ -284,6 +284,43 @@ def leg_film(stream):
             "host_film_ms": round(t_host * 1e3, 2), "bytes_equal": dev_bytes == host_bytes}
 
 
+def host_cpu():
+    """The host CPU the baselines ran on: lscpu's model name, sockets, cores per socket and threads
+    per core, with the CPUs this process may run on."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except OSError:
+        usable = os.cpu_count()
+    return {"model": info.get("Model name", "unknown"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "cpus_online": info.get("CPU(s)"), "cpus_usable": usable}
+
+
+def probe_lib():
+    """The benchmark's vector-memory probe (measure/chunk_probe.hip, built by the package build)."""
+    import ctypes
+
+    from sort_amd import lib
+    path = os.path.join(ROOT, "measure", "libchunk_probe.so")
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib()  # the library loads torch's HIP runtime first; the probe shares it
+    p = ctypes.CDLL(path)
+    p.chunk_probe.restype = ctypes.c_int
+    p.chunk_probe.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.c_void_p]
+    return p
+
+
 def memory_pipeline(n, st, mean_ms, value, stream, width):
     """The ceiling the traversal runs against: each lane reads its own records (an 8-wide node as
     8 consecutive 16-byte chunks, 14 when it has more than four children; a 4-wide node as 7; a
@@ -294,7 +331,6 @@ def memory_pipeline(n, st, mean_ms, value, stream, width):
     chunk counts."""
     import torch
 
-    from sort_amd._lib import probe_lib
     p = probe_lib()
     node_chunks = 14 if width == 8 else 7
     narrow_chunks = 8 if width == 8 else 7
@@ -305,7 +341,7 @@ def memory_pipeline(n, st, mean_ms, value, stream, width):
     rates = {}
     for k in sorted({node_chunks, narrow_chunks, 3}):
         def run():
-            rc = p.sortgpu_probe_chunks(buf.data_ptr(), 8192, k, iters, blocks, out.data_ptr(), stream.cuda_stream)
+            rc = p.chunk_probe(buf.data_ptr(), 8192, k, iters, blocks, out.data_ptr(), stream.cuda_stream)
             if rc != 0:
                 raise RuntimeError(f"probe launch failed: {rc}")
         _, ms, _ = timed(run, 5, 2, stream)
@@ -426,7 +462,8 @@ def main():
         cpu_mrays = n / float(np.median(secs)) / 1e6
         cpu = {"value": round(cpu_mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"all {n} incoherent rays, median of {a.cpu_runs} runs after a warm-up; oracle = "
-                         f"C++ AVX (-mavx -mno-fma) restatement of SORT's Obvh traversal, {threads} threads"}
+                         f"C++ AVX (-mavx -mno-fma) restatement of SORT's Obvh traversal, {threads} threads",
+               "host": host_cpu()}
 
     out = {
         "metric": "incoherent Mrays/s",

@@ -1,5 +1,5 @@
-// Vector-memory return-rate probe (a measurement tool beside the traversal library, not part of
-// the drop-in boundary).
+// Vector-memory return-rate probe: measurement code of the benchmark (measure/libchunk_probe.so),
+// not part of the library or of its drop-in boundary.
 //
 // A traversal lane reads its own record: 14 consecutive 16-byte chunks of a 256-byte node, or 3
 // of a 48-byte triangle record, at addresses no other lane shares. What bounds such a kernel is the
@@ -9,7 +9,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "sortgpu_probe.h"
 
 namespace {
 
@@ -50,8 +49,16 @@ hipError_t launch(const void* buf, uint32_t n_rec, int iters, int blocks, float*
 
 }  // namespace
 
-extern "C" int sortgpu_probe_chunks(const void* buf, uint32_t n_rec, int chunks, int iters, int blocks, float* out,
-                                    void* stream) {
+// Launches `blocks` workgroups of 256 lanes on `stream` (a hipStream_t, NULL = default stream).
+// Every lane reads, `iters` times, `chunks` (1, 3, 7, 8, 14 or 16) consecutive 16-byte chunks of a
+// pseudo-random 256-byte record of `buf` (a device buffer of n_rec * 256 bytes), all chunks of a
+// record issued back to back, and writes one float to out[lane] (blocks * 256 floats). Returns 0
+// or a hipError_t code (other chunk counts are refused); asynchronous.
+extern "C" int chunk_probe(const void* buf, uint32_t n_rec, int chunks, int iters, int blocks, float* out,
+                           void* stream);
+
+extern "C" int chunk_probe(const void* buf, uint32_t n_rec, int chunks, int iters, int blocks, float* out,
+                           void* stream) {
     if (!buf || !out || n_rec == 0 || iters < 1 || blocks < 1) return (int)hipErrorInvalidValue;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     switch (chunks) {

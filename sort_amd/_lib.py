@@ -1,7 +1,7 @@
 """ctypes binding of libsortgpu.so (the C ABI declared in include/sortgpu.h).
 
-The library is loaded from this package directory (or from the path in SORTGPU_LIB). A missing
-library is an error: there is no fallback implementation.
+The library is loaded from this package directory. A missing library is an error: there is no
+fallback implementation.
 """
 from __future__ import annotations
 
@@ -10,8 +10,7 @@ import os
 import re
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-# SORTGPU_LIB names another build of the same library (e.g. a kernel variant being measured).
-LIB_PATH = os.environ.get("SORTGPU_LIB") or os.path.join(PKG, "libsortgpu.so")
+LIB_PATH = os.path.join(PKG, "libsortgpu.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "sortgpu.h")
 
 OK = 0
@@ -173,25 +172,6 @@ def header_symbols(header: str = HEADER) -> list[str]:
     """Every function a header declares (default: the public header)."""
     text = open(header).read()
     return sorted(set(re.findall(r"\b(sortgpu_[a-z0-9_]+)\s*\(", text)))
-
-
-PROBE_LIB_PATH = os.path.join(PKG, "libsortgpu_probe.so")
-PROBE_HEADER = os.path.join(os.path.dirname(PKG), "include", "sortgpu_probe.h")
-_probe = None
-
-
-def probe_lib() -> ctypes.CDLL:
-    """The vector-memory probe library (measurement tool, not the drop-in boundary)."""
-    global _probe
-    if _probe is None:
-        if not os.path.exists(PROBE_LIB_PATH):
-            raise ImportError(f"{PROBE_LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
-        lib()  # same runtime-order rule as the library
-        l = ctypes.CDLL(PROBE_LIB_PATH)
-        l.sortgpu_probe_chunks.restype = _I
-        l.sortgpu_probe_chunks.argtypes = [_P, ctypes.c_uint32, _I, _I, _I, _P, _P]
-        _probe = l
-    return _probe
 
 
 def lib() -> ctypes.CDLL:

@@ -40,16 +40,17 @@ HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp", "pool.cpp", "ser
                 "scene_file.cpp"]
 # compiled as HIP: kernels, and host code that shares device record types with them
 DEVICE_SOURCES = ["trace.hip", "order.hip", "multihit.hip", "film_out.hip", "pt.hip", "pt_host.cpp", "pt_api.cpp", "serve_device.cpp"]
-# the vector-memory probe bench.py measures the traversal's ceiling with (its own library)
-PROBE_SOURCES = ["probe.hip"]
-PROBE_LIB = os.path.join(PKG, "libsortgpu_probe.so")
+# the vector-memory probe bench.py measures the traversal's ceiling with (benchmark code, its own
+# library beside its source; not part of libsortgpu)
+PROBE_SRC = os.path.join(ROOT, "measure", "chunk_probe.hip")
+PROBE_LIB = os.path.join(ROOT, "measure", "libchunk_probe.so")
 
 
 def _compile(src: str) -> str:
     os.makedirs(OBJ, exist_ok=True)
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON
-    if src in DEVICE_SOURCES or src in PROBE_SOURCES:
+    if src in DEVICE_SOURCES or src == PROBE_SRC:
         cmd += DEVICE + ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -68,13 +69,13 @@ def _link(objs: list[str], out: str) -> None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compiles every source and links ``sort_amd/libsortgpu.so`` (and the measurement probe
-    ``sort_amd/libsortgpu_probe.so``); returns the library's path."""
-    srcs = HOST_SOURCES + DEVICE_SOURCES + PROBE_SOURCES
+    """Compiles every source and links ``sort_amd/libsortgpu.so`` (and the benchmark's probe
+    ``measure/libchunk_probe.so``); returns the library's path."""
+    srcs = HOST_SOURCES + DEVICE_SOURCES + [PROBE_SRC]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = dict(zip(srcs, ex.map(_compile, srcs)))
     _link([objs[s] for s in HOST_SOURCES + DEVICE_SOURCES], LIB)
-    _link([objs[s] for s in PROBE_SOURCES], PROBE_LIB)
+    _link([objs[PROBE_SRC]], PROBE_LIB)
     if verbose:
         print(f"built {LIB} and {PROBE_LIB}")
     return LIB

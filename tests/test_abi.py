@@ -1,12 +1,14 @@
 """C-ABI checks that need no GPU: the library loads, exports every declared symbol, rejects
 bad arguments with status codes, and its host-side build equals the oracle's tree."""
 import ctypes
+import subprocess
 
 import numpy as np
 import pytest
 
 from oracle.oracle import OracleBvh
 from sort_amd import Context, Obvh, Qbvh, SortGpuError, header_symbols, lib, scenes
+from sort_amd._lib import LIB_PATH
 
 
 def test_library_exports_every_declared_symbol():
@@ -15,12 +17,19 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 13
     for s in syms:
         assert hasattr(l, s), s
-    # the measurement probe ships as its own library with its own header
-    from sort_amd._lib import PROBE_HEADER, probe_lib
-    p = probe_lib()
-    for s in header_symbols(PROBE_HEADER):
-        assert hasattr(p, s), s
-    assert p.sortgpu_probe_chunks(None, 1, 3, 1, 1, None, None) != 0  # refused before any launch
+    # the library exports nothing beyond the header (no measurement or test hooks)
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = sorted({ln.split()[-1] for ln in out.splitlines() if ln.split()[-1].startswith("sortgpu_")})
+    assert exported == syms
+
+
+def test_benchmark_probe_is_separate():
+    """The benchmark's vector-memory probe is its own library beside bench.py, not part of the
+    drop-in library or its header."""
+    import bench
+    p = bench.probe_lib()
+    assert p.chunk_probe(None, 1, 3, 1, 1, None, None) != 0  # refused before any launch
+    assert not hasattr(lib(), "chunk_probe")
 
 
 def test_host_only_context_and_errors(host_ctx):
