@@ -115,6 +115,39 @@ int sortgpu_trace_closest(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint3
 int sortgpu_trace_any(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
                       uint8_t* d_occluded, void* stream);
 
+/* Single-ray compatibility forms for a shim that answers SORT's per-ray calls
+ * (Accelerator::GetIntersect, accelerator.h:74, and IsOccluded, accelerator.h:85, called by
+ * each worker thread for one ray at a time): host pointers, synchronous, the same traversal and
+ * results as the batched calls. Each calling thread gets its own stream and 64 bytes of device
+ * staging per device on its first call (released when the thread ends), so worker threads may
+ * call concurrently. A wavefront caller uses the batched entry points instead: one round trip per
+ * ray costs tens of microseconds. */
+int sortgpu_trace_closest1(sortgpu_accel* accel, const sortgpu_ray* ray, sortgpu_hit* hit);
+int sortgpu_trace_any1(sortgpu_accel* accel, const sortgpu_ray* ray, int* occluded);
+
+/* What SORT's SurfaceInteraction holds after a closest hit (core/interaction.h), rebuilt on the
+ * host from a hit record. */
+typedef struct sortgpu_surface {
+    float intersect[3]; /* ray(t) = o + t d (math/ray.h:55-57) */
+    float t;
+    float gnormal[3];   /* triangles: normalize(cross(p2 - p0, p1 - p0)); quads: = normal */
+    float normal[3];    /* interpolated shading normal (0 when no normals were given) */
+    float tangent[3];   /* interpolated tangent (0 when no tangents were given) */
+    float view[3];      /* -d */
+    float u, v;         /* interpolated texture coordinates (0 for quads or without texcoords) */
+    uint32_t prim;
+    uint32_t is_quad;
+} sortgpu_surface;
+
+/* Hit reconstruction (setupIntersection, simd/simd_triangle.h:311-341, for triangles; the hit
+ * branch of Quad::GetIntersect, shape/quad.cpp:84-91, for quads) from a closest-hit record of
+ * `accel` and the ray that produced it, with SORT's float operations in SORT's order. normals and
+ * tangents (nverts x 3) and texcoords (nverts x 2) are the mesh's per-vertex attributes in the
+ * build's vertex order (SORT's MeshVertex), each optional. Host only; a miss or a primitive id the
+ * accelerator does not hold is refused with SORTGPU_ERR_ARG. */
+int sortgpu_hit_surface(const sortgpu_accel* accel, const sortgpu_ray* ray, const sortgpu_hit* hit,
+                        const float* normals, const float* tangents, const float* texcoords, sortgpu_surface* out);
+
 /* BSSRDF probe query (Accelerator::GetIntersect with BSSRDFIntersections and a material id,
  * accel/fast_bvh.hpp:656-752, simd/simd_triangle.h:412-452, scatteringevent/bssrdf/bssrdf.h:32-56):
  * up to 4 triangle hits per ray whose triangle has material d_ray_material[i]

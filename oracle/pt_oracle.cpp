@@ -1004,6 +1004,19 @@ static double render_impl(void* p, const float* cam10, uint32_t width, uint32_t 
 
 extern "C" {
 
+// The random-number generator alone: Philox4x32-10 of a counter and key, and the four floats of
+// one (seed, pixel, sample, bounce, call) draw (pinned by the generator's published known-answer
+// vectors in the tests).
+void oracle_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox(c, k0, k1);
+    for (int i = 0; i < 4; i++) out[i] = c[i];
+}
+void oracle_draws(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t call, float* out) {
+    const Draws d = draws(seed, pixel, sample, bounce, call);
+    for (int i = 0; i < 4; i++) out[i] = d.v[i];
+}
+
 // Batched Li over caller rays: keys = (pixel, sample) per ray; out n x 3.
 void oracle_pt_li(void* p, const float* rays8, const uint32_t* keys, uint32_t n, uint32_t max_depth, uint64_t seed64,
                   int nthreads, float* out) {

@@ -37,6 +37,12 @@ class Quad(ctypes.Structure):
                 ("size_x", ctypes.c_float), ("size_y", ctypes.c_float)]
 
 
+class Surface(ctypes.Structure):
+    _fields_ = [("intersect", ctypes.c_float * 3), ("t", ctypes.c_float), ("gnormal", ctypes.c_float * 3),
+                ("normal", ctypes.c_float * 3), ("tangent", ctypes.c_float * 3), ("view", ctypes.c_float * 3),
+                ("u", ctypes.c_float), ("v", ctypes.c_float), ("prim", ctypes.c_uint32), ("is_quad", ctypes.c_uint32)]
+
+
 class BuildOpts(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("max_leaf", ctypes.c_uint32), ("build_threads", ctypes.c_uint32)]
@@ -121,6 +127,9 @@ _SIGS = {
     "sortgpu_accel_prim_order": (_I, [_P, _P, ctypes.c_uint64]),
     "sortgpu_accel_tree_dump": (ctypes.c_int64, [_P, _P, ctypes.c_uint64]),
     "sortgpu_trace_closest": (_I, [_P, _P, _U32, _P, _P]),
+    "sortgpu_trace_closest1": (_I, [_P, _P, _P]),
+    "sortgpu_trace_any1": (_I, [_P, _P, ctypes.POINTER(_I)]),
+    "sortgpu_hit_surface": (_I, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(Surface)]),
     "sortgpu_trace_any": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_attenuation": (_I, [_P, _P, _U32, _P, _U32, _P, _P]),
     "sortgpu_trace_multi": (_I, [_P, _P, _U32, _P, _P, _U32, _P, _P, _P]),

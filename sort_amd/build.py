@@ -37,7 +37,7 @@ DEVICE = [
 ]
 
 HOST_SOURCES = ["api.cpp", "accel_device.cpp", "bvh_build.cpp", "pool.cpp", "serve_engine.cpp", "serve_api.cpp",
-                "scene_file.cpp"]
+                "scene_file.cpp", "hit_surface.cpp"]
 # compiled as HIP: kernels, and host code that shares device record types with them
 DEVICE_SOURCES = ["trace.hip", "order.hip", "multihit.hip", "film_out.hip", "pt.hip", "pt_host.cpp", "pt_api.cpp", "serve_device.cpp"]
 # the vector-memory probe bench.py measures the traversal's ceiling with (benchmark code, its own

@@ -387,6 +387,96 @@ int sortgpu_trace_any(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n,
     return trace_common(acc, d_rays, n, d_occluded, true, false, nullptr, stream);
 }
 
+}  // extern "C"
+
+namespace {
+// Per-thread staging for the single-ray queries: a stream and 64 bytes of device memory (ray,
+// hit, occlusion byte) per device the thread queries, created on its first single-ray call and
+// released when the thread ends. SORT's worker threads each get their own, so concurrent calls
+// never share a buffer.
+struct RayStage {
+    int device = -1;
+    hipStream_t s = nullptr;
+    unsigned char* d = nullptr;
+};
+struct RayStages {
+    RayStage st[8];
+    ~RayStages() {
+        for (RayStage& r : st) {
+            if (r.device < 0) continue;
+            (void)hipSetDevice(r.device);
+            if (r.d) (void)hipFree(r.d);
+            if (r.s) (void)hipStreamDestroy(r.s);
+        }
+    }
+};
+thread_local RayStages t_stages;
+
+int single_ray(sortgpu_accel* acc, const sortgpu_ray* ray, bool any, sortgpu_hit* hit, int* occluded) {
+    if (!acc || !ray || (!any && !hit) || (any && !occluded)) {
+        set_error(acc ? acc->ctx : nullptr, "trace (single ray): null argument");
+        return SORTGPU_ERR_ARG;
+    }
+    sortgpu_ctx* ctx = acc->ctx;
+    if (ctx->host_only) {
+        set_error(ctx, "trace (single ray): host-only context has no device");
+        return SORTGPU_ERR_STATE;
+    }
+    int rc = ctx_bind(ctx);
+    if (rc != SORTGPU_OK) return rc;
+    RayStage* stage = nullptr;
+    for (RayStage& r : t_stages.st)
+        if (r.device == ctx->device) {
+            stage = &r;
+            break;
+        }
+    if (!stage) {
+        for (RayStage& r : t_stages.st)
+            if (r.device < 0) {
+                stage = &r;
+                break;
+            }
+        if (!stage) {
+            set_error(ctx, "trace (single ray): one thread queries more than 8 devices");
+            return SORTGPU_ERR_STATE;
+        }
+        hipError_t e = hipStreamCreateWithFlags(&stage->s, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamCreate(single ray)");
+        void* p = nullptr;
+        if ((e = hipMalloc(&p, 64)) != hipSuccess) {
+            (void)hipStreamDestroy(stage->s);
+            stage->s = nullptr;
+            return hip_fail(ctx, e, "hipMalloc(single ray)");
+        }
+        stage->d = static_cast<unsigned char*>(p);
+        stage->device = ctx->device;
+    }
+    hipError_t e = hipMemcpyAsync(stage->d, ray, sizeof(sortgpu_ray), hipMemcpyHostToDevice, stage->s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync(single ray)");
+    void* d_out = stage->d + 32;
+    if ((rc = trace_common(acc, reinterpret_cast<const sortgpu_ray*>(stage->d), 1, d_out, any, false, nullptr,
+                           stage->s)) != SORTGPU_OK)
+        return rc;
+    uint8_t occ = 0;
+    e = any ? hipMemcpyAsync(&occ, d_out, 1, hipMemcpyDeviceToHost, stage->s)
+            : hipMemcpyAsync(hit, d_out, sizeof(sortgpu_hit), hipMemcpyDeviceToHost, stage->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(stage->s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "single ray result");
+    if (any) *occluded = occ ? 1 : 0;
+    return SORTGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sortgpu_trace_closest1(sortgpu_accel* acc, const sortgpu_ray* ray, sortgpu_hit* hit) {
+    return single_ray(acc, ray, false, hit, nullptr);
+}
+
+int sortgpu_trace_any1(sortgpu_accel* acc, const sortgpu_ray* ray, int* occluded) {
+    return single_ray(acc, ray, true, nullptr, occluded);
+}
+
 int sortgpu_trace_attenuation(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, const float* d_transparency,
                               uint32_t n_transparency, float* d_atten, void* stream) {
     if (!acc) {

@@ -54,6 +54,10 @@ static void layouts(void) {
     F(sortgpu_request_info, state); F(sortgpu_request_info, status); F(sortgpu_request_info, samples_done);
     F(sortgpu_request_info, spp); F(sortgpu_request_info, t_submit); F(sortgpu_request_info, t_admit);
     F(sortgpu_request_info, t_done);
+    S(sortgpu_surface);
+    F(sortgpu_surface, intersect); F(sortgpu_surface, t); F(sortgpu_surface, gnormal); F(sortgpu_surface, normal);
+    F(sortgpu_surface, tangent); F(sortgpu_surface, view); F(sortgpu_surface, u); F(sortgpu_surface, v);
+    F(sortgpu_surface, prim); F(sortgpu_surface, is_quad);
     S(sortgpu_scene_file_info);
     F(sortgpu_scene_file_info, threads); F(sortgpu_scene_file_info, spp); F(sortgpu_scene_file_info, image_width);
     F(sortgpu_scene_file_info, image_height); F(sortgpu_scene_file_info, clamping);

@@ -24,6 +24,7 @@ CTYPES = {
     "sortgpu_material": _lib.Material, "sortgpu_scene_desc": _lib.SceneDesc, "sortgpu_camera": _lib.Camera,
     "sortgpu_server_opts": _lib.ServerOpts, "sortgpu_request": _lib.Request,
     "sortgpu_request_info": _lib.RequestInfo, "sortgpu_scene_file_info": _lib.SceneFileInfo,
+    "sortgpu_surface": _lib.Surface,
 }
 
 
