@@ -109,6 +109,17 @@ int64_t sortgpu_accel_tree_dump(const sortgpu_accel* accel, uint32_t* out, uint6
 int sortgpu_trace_closest(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
                           sortgpu_hit* d_hits, void* stream);
 
+/* sortgpu_trace_closest for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Same results and traversal order; the order buffers of a large batch live in an 80 MiB array
+ * the library holds on each device it runs on (until the process ends) instead of being
+ * allocated per call, and the call makes no allocation of any kind. n is at most 4,194,304 rays
+ * (larger batches are refused with SORTGPU_ERR_ARG: split them). Every call on one device shares
+ * that device's array: calls that overlap on one device (several streams or threads) race and
+ * must be serialised by the caller. */
+int sortgpu_trace_closest_noalloc(sortgpu_accel* accel, const sortgpu_ray* d_rays, uint32_t n,
+                                  sortgpu_hit* d_hits, void* stream);
+
 /* Any hit / occlusion for n rays (Accelerator::IsOccluded, accel/accelerator.h:85; live form:
  * GetIntersect with query_shadow = true, fast_bvh.hpp:362-373, accelerator.cpp:26-39).
  * d_occluded[i] = 1 when some primitive is hit with tmin < t <= tmax, else 0. */

@@ -127,6 +127,7 @@ _SIGS = {
     "sortgpu_accel_prim_order": (_I, [_P, _P, ctypes.c_uint64]),
     "sortgpu_accel_tree_dump": (ctypes.c_int64, [_P, _P, ctypes.c_uint64]),
     "sortgpu_trace_closest": (_I, [_P, _P, _U32, _P, _P]),
+    "sortgpu_trace_closest_noalloc": (_I, [_P, _P, _U32, _P, _P]),
     "sortgpu_trace_closest1": (_I, [_P, _P, _P]),
     "sortgpu_trace_any1": (_I, [_P, _P, ctypes.POINTER(_I)]),
     "sortgpu_hit_surface": (_I, [_P, _P, _P, _P, _P, _P, ctypes.POINTER(Surface)]),

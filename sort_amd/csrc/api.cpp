@@ -382,6 +382,54 @@ int sortgpu_trace_closest(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_
     return trace_common(acc, d_rays, n, d_hits, false, false, nullptr, stream);
 }
 
+// sortgpu_trace_closest without any allocation: the traversal order of a large batch is sorted in
+// the library's file-scope device array (order.hip) instead of pool memory. The work counter and
+// error word are the accelerator's own, allocated when it was built.
+int sortgpu_trace_closest_noalloc(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, sortgpu_hit* d_hits,
+                                  void* stream) {
+    if (!acc) {
+        set_error(nullptr, "trace_closest_noalloc: accel is NULL");
+        return SORTGPU_ERR_ARG;
+    }
+    sortgpu_ctx* ctx = acc->ctx;
+    if (ctx->host_only) {
+        set_error(ctx, "trace_closest_noalloc: host-only context has no device");
+        return SORTGPU_ERR_STATE;
+    }
+    if (n > kScratchRays) {
+        set_error(ctx, "trace_closest_noalloc: %u rays exceed the no-allocation limit of %u per call; split the batch",
+                  n, kScratchRays);
+        return SORTGPU_ERR_ARG;
+    }
+    if (n == 0) return SORTGPU_OK;
+    if (!d_rays || !d_hits) {
+        set_error(ctx, "trace_closest_noalloc: null ray or output pointer");
+        return SORTGPU_ERR_ARG;
+    }
+    int rc = ctx_bind(ctx);
+    if (rc != SORTGPU_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    TraceArgs a{};
+    a.tree = acc->view;
+    a.rays = d_rays;
+    a.n = n;
+    a.out = d_hits;
+    a.counter = acc->d_counters + (acc->slot.fetch_add(1) % kCounterSlots) * kCounterStride;
+    a.error = acc->d_error;
+    apply_tuning(ctx, a);
+    const uint32_t reorder = ctx->tuning.reorder;
+    if (reorder == 2 || (reorder != 1 && n >= kReorderMinRays)) {
+        void* base = nullptr;
+        hipError_t e = scratch_base(ctx->device, &base);
+        if (e != hipSuccess) return hip_fail(ctx, e, "trace_closest_noalloc: scratch address");
+        e = order_rays_scratch(base, d_rays, n, acc->view.bbox, s, &a.order);
+        if (e != hipSuccess) return hip_fail(ctx, e, "trace_closest_noalloc: traversal order");
+    }
+    hipError_t e = launch_trace(a, false, false, s, ctx->num_cus);
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace_closest_noalloc launch");
+    return SORTGPU_OK;
+}
+
 int sortgpu_trace_any(sortgpu_accel* acc, const sortgpu_ray* d_rays, uint32_t n, uint8_t* d_occluded,
                       void* stream) {
     return trace_common(acc, d_rays, n, d_occluded, true, false, nullptr, stream);

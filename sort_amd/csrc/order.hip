@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "trace_launch.h"
 
 namespace sortgpu {
@@ -87,6 +89,79 @@ hipError_t order_rays(hipMemPool_t pool, const void* rays, uint32_t n, const flo
     }
     ro.order = vb.Current();
     return hipSuccess;
+}
+
+// ---- traversal order without allocation -------------------------------------------------------
+// A file-scope device array holds the order buffers (keys and indices, double-buffered, and the
+// radix sort's workspace) for batches of up to kScratchRays rays, so callers that cannot allocate
+// device memory (code under graph capture, serving loops, harnesses that forbid hipMalloc) get the
+// same traversal order. Each device holds its own copy for the life of the process; every call on
+// a device shares it, so such calls that overlap on one device race.
+namespace {
+constexpr size_t kScratchArray = (size_t)kScratchRays * sizeof(uint32_t);
+constexpr size_t kScratchTemp = kScratchBytes - 4 * kScratchArray;
+static_assert(kScratchBytes > 4 * kScratchArray, "scratch must hold the four order arrays and a workspace");
+static_assert(kScratchArray % 256 == 0, "order arrays stay 256-byte aligned");
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t scratch_base(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // the device must be current: each device holds the array at its own address
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+hipError_t order_rays_scratch(void* base, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
+                              const uint32_t** order) {
+    *order = nullptr;
+    if (!base || n > kScratchRays) return hipErrorInvalidValue;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    uint32_t* keys0 = reinterpret_cast<uint32_t*>(b);
+    uint32_t* keys1 = reinterpret_cast<uint32_t*>(b + kScratchArray);
+    uint32_t* vals0 = reinterpret_cast<uint32_t*>(b + 2 * kScratchArray);
+    uint32_t* vals1 = reinterpret_cast<uint32_t*>(b + 3 * kScratchArray);
+    void* temp = b + 4 * kScratchArray;
+    float ext[3];
+    for (int k = 0; k < 3; k++) {
+        const float w = bbox[3 + k] - bbox[k];
+        ext[k] = w > 0.0f ? 1.0f / w : 0.0f;
+    }
+    hipcub::DoubleBuffer<uint32_t> kb(keys0, keys1), vb(vals0, vals1);
+    size_t tb = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (int)n, 0, kKeyBits, s);
+    if (e != hipSuccess) return e;
+    if (tb > kScratchTemp) return hipErrorInvalidValue;  // the workspace does not fit: refuse, never overrun
+    hipLaunchKernelGGL(k_ray_keys, dim3((n + 255) / 256), dim3(256), 0, s, static_cast<const float4*>(rays), n, bbox[0],
+                       bbox[1], bbox[2], ext[0], ext[1], ext[2], keys0, vals0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(temp, tb, kb, vb, (int)n, 0, kKeyBits, s)) != hipSuccess) return e;
+    *order = vb.Current();
+    return hipSuccess;
+}
+
+size_t order_scratch_temp_bytes(uint32_t n) {
+    hipcub::DoubleBuffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
+    size_t tb = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (int)n, 0, kKeyBits, nullptr) != hipSuccess) return 0;
+    return tb;
 }
 
 }  // namespace sortgpu

@@ -70,6 +70,19 @@ struct RayOrder {
 hipError_t order_rays(hipMemPool_t pool, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
                       RayOrder& ro);
 void release_order(RayOrder& ro, hipStream_t s);
+// Traversal order without allocation (the no-allocation entry point): buffers for batches of up
+// to kScratchRays rays live in a file-scope device array of kScratchBytes per device.
+constexpr uint32_t kScratchRays = 1u << 22;
+constexpr size_t kScratchBytes = (size_t)80 << 20;  // 4 x 16 MiB order arrays + 16 MiB sort workspace
+constexpr int kMaxScratchDevices = 64;
+// Device address of the calling thread's current device's array (taken once per device with
+// hipGetSymbolAddress; `device` must be current).
+hipError_t scratch_base(int device, void** out);
+// order_rays on the array at `base` (n <= kScratchRays); *order points into it.
+hipError_t order_rays_scratch(void* base, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
+                              const uint32_t** order);
+// Radix-sort workspace bytes a batch of n rays needs (0 if the query fails).
+size_t order_scratch_temp_bytes(uint32_t n);
 // Error flags the traversal kernels raise (atomicOr into a 16-byte device word block).
 constexpr uint32_t kFlagStackOverflow = 1u;  // a push found the stack full: that ray's result is incomplete
 constexpr uint32_t kFlagBadScene = 2u;       // multi-scene form: a ray named a tree outside the table
