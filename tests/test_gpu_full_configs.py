@@ -116,10 +116,11 @@ def test_served_poisson_trace_with_evictions(gpu_ctx):
     probe.close()
     sizes = np.array(sizes)
     films = 4 * 1024 * 1024 * 16
-    # room for the largest scene and two median ones: every request fits alone, but the trace's
-    # scenes (9 of the 16) do not fit together, so least-recently-used ones must be evicted
-    budget = int(fixed + sizes.max() + 2 * np.median(sizes) + films)
-    assert budget < fixed + np.sort(sizes)[-4:].sum()
+    # room for half of the scenes the trace uses (at least its largest): every request fits
+    # alone, but its scenes do not fit together, so least-recently-used ones must be evicted
+    used = sizes[sorted({r["scene"] for r in trace})]
+    budget = int(fixed + films + max(used.max(), used.sum() // 2))
+    assert len(used) >= 4 and budget < fixed + films + used.sum()
     srv = S.Server(gpu_ctx, hbm_budget=budget, wave_paths=1 << 23, max_active=4)
     ids = S.add_scenes(srv, built)
     print(f"[full] served: 20 Poisson requests, budget {budget} ({len(built)} scenes, {sizes.sum()} scene bytes)",
