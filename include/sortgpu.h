@@ -216,13 +216,15 @@ int sortgpu_accel_errors(sortgpu_accel* accel, void* stream, uint32_t* flags);
 
 /* ---- tuning (no counterpart in SORT: the knobs of the persistent traversal kernels) ---- */
 
-/* 0 in any field keeps the default. refill = idle lanes of a wave that trigger taking new rays
- * (default 16, 1..64); lds_entries = traversal-stack entries per lane kept in LDS (8 or 16,
+/* 0 in any field keeps the default. refill = the compaction threshold of the persistent waves:
+ * idle lanes of a wave that trigger refilling them with new rays from the work queue (default 16,
+ * 1..64); lds_entries = traversal-stack entries per lane kept in LDS (8 or 16,
  * default 16); blocks_per_cu = cap on resident workgroups per CU (default: occupancy limit);
  * leaf_batch = lanes of a wave holding a leaf that end a node-walking phase (default 32, 1..64);
  * leaf_min = a leaf-testing phase ends when fewer lanes hold a leaf (default 1, 1..64);
- * reorder = traversal order of sortgpu_trace_closest / _any / _attenuation / _multi batches: 0 (default)
- * sorts batches of at least 262,144 rays by origin cell (64^3 grid over the scene box) before the
+ * reorder = traversal order of sortgpu_trace_closest / _any / _attenuation / _multi and
+ * sortgpu_pool_trace batches: 0 (default) sorts batches of at least 262,144 rays by origin cell
+ * (64^3 grid over the scene box; pooled rays by scene, then by cell in their scene's box) before the
  * launch (stream-ordered device memory of 16 bytes per ray plus sort workspace, from a pool the
  * context owns; a batch sharing one origin, such as camera rays, keeps its order and pays only the
  * sort), 1 = never, 2 = always. The knobs change speed only: every hit is the same for every
@@ -265,6 +267,9 @@ int sortgpu_pool_errors(sortgpu_pool* pool, void* stream, uint32_t* flags);
 #define SORTGPU_MAT_GGX 1        /* MicroFacetReflection, GGX, conductor Fresnel (microfacet.cpp)  */
 #define SORTGPU_MAT_DIELECTRIC 2 /* rough dielectric, exterior IOR 1 (dielectric.cpp)              */
 #define SORTGPU_MAT_DISNEY 3     /* Disney BRDF subset: no transmission / thin / SSS (disney.cpp)  */
+#define SORTGPU_MAT_TRANSPARENT 4 /* Transparent (bsdf/transparent.cpp): base = attenuation; light
+                                     passes straight through, and a scene holding one answers its
+                                     shadow rays with the transparent-shadow walk (GetAttenuation) */
 
 /* One BXDF per material (the configurations' materials; SORT builds them from TSL closures). */
 typedef struct sortgpu_material {

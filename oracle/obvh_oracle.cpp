@@ -1179,6 +1179,19 @@ int oracle_ray1(OracleHandle* h, const void* ray_, void* hit_, int mode) {
     return res ? 1 : 0;
 }
 
+// Transparent-shadow attenuation of one ray on the calling thread (the path tracer's shadow
+// queries in a scene with transparent materials): out = (r, g, b, surfaces passed).
+void oracle_atten1(OracleHandle* h, const void* ray_, const float* transp, float* out, uint32_t max_steps) {
+    const RayRec* r = static_cast<const RayRec*>(ray_);
+    if (h->width == 8) {
+        thread_local std::vector<std::pair<const Node<8>*, float>> st;
+        B<8>(h)->Attenuation(toRay(*r), transp, out, st, max_steps);
+    } else {
+        thread_local std::vector<std::pair<const Node<4>*, float>> st;
+        B<4>(h)->Attenuation(toRay(*r), transp, out, st, max_steps);
+    }
+}
+
 // Transparent-shadow attenuation of n rays: transp = 4 floats per primitive id; out = 4 floats per
 // ray (r, g, b, surfaces passed). Walks stop after max_steps surfaces.
 void oracle_attenuation(OracleHandle* h, const void* rays_, uint64_t n, const float* transp, float* out,

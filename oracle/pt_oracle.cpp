@@ -502,6 +502,20 @@ struct DisneyBRDF : Bxdf {  // disney.cpp:181-434, subset
     }
 };
 
+// Transparent (bsdf/transparent.cpp:24-37): no reflection; sampling passes straight through
+// with the attenuation A and pdf 1.
+struct Transparent : Bxdf {
+    Spectrum A;
+    explicit Transparent(Spectrum a) : A(a) {}
+    Spectrum f(Vec, Vec) const override { return Spectrum(0.0f); }
+    float pdf(Vec, Vec) const override { return 0.0f; }
+    Spectrum sample_f(Vec wo, Vec& wi, const BsdfSample&, float* pdf) const override {
+        wi = -wo;
+        if (pdf) *pdf = 1.0f;
+        return A;
+    }
+};
+
 // Material record (same field order as the product's sortgpu_material).
 struct MatIn {
     uint32_t type;
@@ -515,6 +529,7 @@ struct BxdfHolder {
     FresnelConductor conductor{Spectrum(1.0f), Spectrum(0.0f)};
     GGX ggx{0.5f};
     MicroFacetReflection mfr{Spectrum(1.0f), &conductor, &ggx, false};
+    Transparent transparent{Spectrum(1.0f)};
     const Bxdf* bxdf = nullptr;
     std::unique_ptr<Dielectric> diel;
     std::unique_ptr<DisneyBRDF> disney;
@@ -541,6 +556,10 @@ static void make_bxdf(const MatIn* m, BxdfHolder& h) {
             h.diel.reset(new Dielectric(base, Spectrum(m->trans[0], m->trans[1], m->trans[2]), m->roughness,
                                         m->ior > 0.0f ? m->ior : 1.5f));
             h.bxdf = h.diel.get();
+            break;
+        case 4:
+            h.transparent = Transparent(base);
+            h.bxdf = &h.transparent;
             break;
         default:
             h.disney.reset(new DisneyBRDF(base, m->metallic, m->specular, m->specular_tint, m->roughness, m->sheen,
@@ -625,6 +644,7 @@ struct QuadLight {
 extern "C" {
 struct OracleHandle;
 int oracle_ray1(OracleHandle* h, const void* ray, void* hit, int mode);
+void oracle_atten1(OracleHandle* h, const void* ray, const float* transp, float* out, uint32_t max_steps);
 }
 
 namespace pto {
@@ -643,6 +663,9 @@ struct PtScene {
     float bbox[6];
     uint32_t ntris = 0;
     float clamp = 0.0f;  // > 0: ImageEvaluation's per-sample clamp (image_evaluation.cpp:222-226)
+    // Material::EvaluateTransparency per primitive id (r, g, b, has-transparency), when some
+    // material is transparent: shadow rays then take Scene::GetAttenuation (core/scene.cpp:192-208)
+    std::vector<float> transp;
 };
 
 struct RayRec {
@@ -662,6 +685,13 @@ static bool closest(const PtScene& sc, Vec o, Vec d, float tmin, float tmax, Hit
 static bool occluded(const PtScene& sc, Vec o, Vec d, float tmin, float tmax) {
     RayRec r{o.x, o.y, o.z, tmin, d.x, d.y, d.z, tmax};
     return oracle_ray1(sc.bvh, &r, nullptr, 1) != 0;
+}
+// Visibility::GetAttenuation (light/light.h:56-58): Scene::GetAttenuation along the shadow ray.
+static Spectrum attenuation(const PtScene& sc, Vec o, Vec d, float tmin, float tmax) {
+    RayRec r{o.x, o.y, o.z, tmin, d.x, d.y, d.z, tmax};
+    float out[4];
+    oracle_atten1(sc.bvh, &r, sc.transp.data(), out, 4096u);
+    return Spectrum(out[0], out[1], out[2]);
 }
 
 static int SampleDiscrete(const PtScene& sc, float u, float* pdf) {  // samplemethod.h:158-176
@@ -741,10 +771,19 @@ static Spectrum EvaluateDirect(const PtScene& sc, const ScatteringEvent& se, Vec
         const Spectrum f = se.Evaluate_BSDF(wo, wi);
         if (!f.IsBlack()) {
             shadow_rays++;
-            if (!occluded(sc, ip, wi, 0.01f, vis_max)) {
-                const float bsdf_pdf = se.Pdf_BSDF(wo, wi);
-                const float weight = MisFactor(light_pdf, bsdf_pdf);
-                radiance = li * f * weight / light_pdf;
+            if (sc.transp.empty()) {
+                if (!occluded(sc, ip, wi, 0.01f, vis_max)) {
+                    const float bsdf_pdf = se.Pdf_BSDF(wo, wi);
+                    const float weight = MisFactor(light_pdf, bsdf_pdf);
+                    radiance = li * f * weight / light_pdf;
+                }
+            } else {  // the ENABLE_TRANSPARENT_SHADOW branch (integratormethod.cpp:121-145)
+                const Spectrum att = attenuation(sc, ip, wi, 0.01f, vis_max);
+                if (!att.IsBlack()) {
+                    const float bsdf_pdf = se.Pdf_BSDF(wo, wi);
+                    const float weight = MisFactor(light_pdf, bsdf_pdf);
+                    radiance = att * li * f * weight / light_pdf;
+                }
             }
         }
     }
@@ -776,7 +815,12 @@ static Spectrum EvaluateDirect(const PtScene& sc, const ScatteringEvent& se, Vec
         if (!hit) return radiance;
         if (!li2.IsBlack()) {
             shadow_rays++;
-            if (!occluded(sc, ip, wi2, 0.001f, t_light - 0.001f)) radiance = radiance + li2 * f * weight / bsdf_pdf;
+            if (sc.transp.empty()) {
+                if (!occluded(sc, ip, wi2, 0.001f, t_light - 0.001f)) radiance = radiance + li2 * f * weight / bsdf_pdf;
+            } else {  // integratormethod.cpp:168-186
+                const Spectrum att = attenuation(sc, ip, wi2, 0.001f, t_light - 0.001f);
+                if (!att.IsBlack()) radiance = radiance + att * li2 * f * weight / bsdf_pdf;
+            }
         }
     }
     return radiance;
@@ -878,6 +922,19 @@ void* oracle_pt_create(OracleHandle* bvh, const float* pos, uint32_t nverts, con
     if (tri_mat) sc->tri_mat.assign(tri_mat, tri_mat + ntris);
     const MatIn* m = static_cast<const MatIn*>(mats);
     sc->mats.assign(m, m + nmats);
+    // transparency per primitive id: a Transparent material's attenuation; every other material
+    // and the quads' default white Lambert have no transparent node (EvaluateTransparency = 0)
+    bool any_transparent = false;
+    for (const MatIn& mi : sc->mats) any_transparent |= mi.type == 4;
+    if (any_transparent) {
+        sc->transp.assign(4 * ((size_t)ntris + nquads), 0.0f);
+        for (uint32_t t = 0; t < ntris; t++) {
+            const MatIn& mi = sc->mats[tri_mat ? tri_mat[t] : 0];
+            if (mi.type != 4) continue;
+            for (int c = 0; c < 3; c++) sc->transp[4 * (size_t)t + c] = mi.base[c];
+            sc->transp[4 * (size_t)t + 3] = 1.0f;
+        }
+    }
     std::vector<float> power;
     for (uint32_t q = 0; q < nquads; q++) {
         QuadLight l;

@@ -73,12 +73,14 @@ class SortFileWriter:
 DIFFUSE, GGX_NODE, GLASS, DISNEY_NODE, OTHER = ("SORTNode_Material_DiffuseLambert", "SORTNode_Material_MicrofacetReflection",
                                                 "SORTNode_Material_Glass", "SORTNode_Material_DisneyBRDF",
                                                 "SORTNode_Material_Cloth")
+TRANSPARENT_NODE = "SORTNode_Material_Transparent"
 _SOURCES = {  # TSL bodies stand-ins: only the closure names the loader inspects matter
     DIFFUSE: "shader bxdf_lambert(color Diffuse, vector Normal, out closure Result){ Result = make_closure<lambert>( Diffuse , Normal ); }",
     GGX_NODE: "shader bxdf_ggx(...){ Result = make_closure<microfacet_reflection_ggx>( InteriorIOR , AbsorptionCoefficient , RoughnessU , RoughnessV , BaseColor , Normal ); }",
     GLASS: "shader bxdf_glass(...){ Result = make_closure<dieletric>( Reflectance , Transmittance , RoughnessU , RoughnessV , Normal ); }",
     DISNEY_NODE: "shader bxdf_disney(...){ Result = make_closure<disney>( Metallic , Specular , SpecularTint , Roughness , Anisotropic , Sheen , SheenTint , Clearcoat , ClearcoatGlossiness , SpecularTransmittance , ScatterDistance , Flatness , DiffuseTransmittance , false , BaseColor , Normal ); }",
     OTHER: "shader bxdf_cloth(...){ }",
+    TRANSPARENT_NODE: "shader bxdf_transparent(color Attenuation, out closure Result){ Result = make_closure<transparent>( Attenuation ); }",
     "SORTNodeOutput": "shader output(closure Surface, closure Volume){ }",
 }
 
@@ -106,6 +108,8 @@ def _node_params(kind, p):
                  "ClearcoatGlossiness", "SpecularTransmittance", "ScatterDistance", "Flatness",
                  "DiffuseTransmittance", "BaseColor"]
         return [(k, d[k]) for k in order] + [("Normal", (0.0, 0.0, 1.0))]
+    if kind == TRANSPARENT_NODE:
+        return [("Attenuation", p.get("Attenuation", (1.0, 1.0, 1.0)))]
     return [("Scale", p.get("Scale", 1.0))]
 
 
@@ -211,7 +215,8 @@ def write_scene(config=None, materials=(), camera=None, meshes=(), lights=(), sk
             for x in c:
                 w.s(x)
         w.sid("Invalid Volume Shader")
-        w.b8(m.get("transparent", False))
+        # the exporter sets the flag when the graph holds a transparent node (exporter.py:753-772,926-927)
+        w.b8(m.get("transparent", kind == TRANSPARENT_NODE))
         w.b8(m.get("sss", False))
         w.f32(0.1)
         w.u32(1024)
@@ -408,7 +413,7 @@ def _graph(r):
     return nodes, conns
 
 
-LAMBERT, GGX, DIELECTRIC, DISNEY = 0, 1, 2, 3
+LAMBERT, GGX, DIELECTRIC, DISNEY, TRANSPARENT = 0, 1, 2, 3, 4
 
 
 def _white():
@@ -435,7 +440,7 @@ def _map(mat, sources):
                     bxdf = n
     if bxdf is None:
         return out, "unmapped"
-    approx = mat["transparent"] or mat["sss"]
+    approx = (mat["transparent"] and bxdf[1] != "SORTNode_Material_Transparent") or mat["sss"]
     if any(c[2] == bxdf[0] and c[3] != "Normal" for c in conns):
         approx = True
     params = bxdf[2]
@@ -471,6 +476,8 @@ def _map(mat, sources):
         if (f1("Anisotropic", 0.0) != 0 or f1("SpecularTransmittance", 0.0) != 0 or f1("DiffuseTransmittance", 0.0) != 0
                 or max(sd) > 0 or src is None or ", true ," in src):
             approx = True
+    elif t == "SORTNode_Material_Transparent":
+        out.update(type=TRANSPARENT, base=f3("Attenuation", 1.0))
     else:
         return out, "unmapped"
     return out, ("approx" if approx else "mapped")

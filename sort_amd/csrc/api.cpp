@@ -281,14 +281,12 @@ int64_t sortgpu_accel_tree_dump(const sortgpu_accel* acc, uint32_t* out, uint64_
     return (int64_t)d.size();
 }
 
-// Batches at least this large are traced in coherence order by default (smaller ones do not pay
-// for the sort).
-constexpr uint32_t kReorderMinRays = 1u << 18;
+}  // extern "C"
 
 // The context's pool for traversal-order buffers: created on first use, on the context's device,
 // keeping freed memory for the next batch (release threshold at its maximum). Null if the device
 // has no stream-ordered allocator.
-static hipMemPool_t order_pool(sortgpu_ctx* ctx) {
+hipMemPool_t sortgpu::order_pool(sortgpu_ctx* ctx) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (!ctx->order_pool) {
         hipMemPoolProps p{};
@@ -306,6 +304,8 @@ static hipMemPool_t order_pool(sortgpu_ctx* ctx) {
     }
     return ctx->order_pool;
 }
+
+extern "C" {
 
 // Coherence order for a large batch on `s` (the tuning's reorder knob decides), or null for index
 // order. An order that cannot be had (no pool, no memory) only costs speed, so it is not an error.

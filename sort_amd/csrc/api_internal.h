@@ -65,6 +65,11 @@ int upload_accel(sortgpu_accel* acc);
 // Copies the context's tuning knobs into a launch.
 void apply_tuning(const sortgpu_ctx* ctx, struct TraceArgs& a);
 void free_accel_device(sortgpu_accel* acc);
+// Batches at least this large are traced in coherence order by default (smaller ones do not pay
+// for the sort).
+constexpr uint32_t kReorderMinRays = 1u << 18;
+// The context's stream-ordered pool for traversal-order buffers (made on first use), or null.
+hipMemPool_t order_pool(sortgpu_ctx* ctx);
 // Takes (and clears) the error flags at d_flags after the work queued on `s`; returns SORTGPU_OK
 // when none was raised, else SORTGPU_ERR_STATE with a message naming each flag ("who: ...").
 int report_flags(sortgpu_ctx* ctx, uint32_t* d_flags, hipStream_t s, const char* who, uint32_t* flags_out);

@@ -48,6 +48,32 @@ __global__ __launch_bounds__(256) void k_ray_keys(const float4* __restrict__ ray
 
 constexpr int kKeyBits = 18;
 
+// Multi-scene form: the scene index above the origin cell in that scene's own box, so the rays of
+// one scene stay together and, within it, walk in cell order. Out-of-range scene ids sort last
+// (their rays miss whatever the order).
+__global__ __launch_bounds__(256) void k_ray_keys_multi(const float4* __restrict__ rays,
+                                                        const uint32_t* __restrict__ ray_scene, uint32_t n,
+                                                        const TreeView* __restrict__ views, uint32_t n_views,
+                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t sid = ray_scene[i];
+    uint32_t key = 0xffffffffu;
+    if (sid < n_views) {
+        const float* b = views[sid].bbox;
+        const float4 q0 = rays[2 * (size_t)i];
+        float ext[3];
+        for (int k = 0; k < 3; k++) {
+            const float w = b[3 + k] - b[k];
+            ext[k] = w > 0.0f ? 1.0f / w : 0.0f;
+        }
+        key = (sid << kKeyBits) | spread3(cell((q0.x - b[0]) * ext[0], 64.f)) |
+              (spread3(cell((q0.y - b[1]) * ext[1], 64.f)) << 1) | (spread3(cell((q0.z - b[2]) * ext[2], 64.f)) << 2);
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
 }  // namespace
 
 void release_order(RayOrder& ro, hipStream_t s) {
@@ -84,6 +110,41 @@ hipError_t order_rays(hipMemPool_t pool, const void* rays, uint32_t n, const flo
     if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (int)n, 0, kKeyBits, s)) != hipSuccess ||
         (e = hipMallocFromPoolAsync(&ro.temp, tb ? tb : 1, pool, s)) != hipSuccess ||
         (e = hipcub::DeviceRadixSort::SortPairs(ro.temp, tb, kb, vb, (int)n, 0, kKeyBits, s)) != hipSuccess) {
+        release_order(ro, s);
+        return e;
+    }
+    ro.order = vb.Current();
+    return hipSuccess;
+}
+
+hipError_t order_rays_multi(hipMemPool_t pool, const void* rays, const uint32_t* ray_scene, uint32_t n,
+                            const TreeView* views, uint32_t n_views, hipStream_t s, RayOrder& ro) {
+    ro = RayOrder{};
+    if (n_views > (1u << (32 - kKeyBits)) - 1) return hipErrorInvalidValue;
+    int bits = kKeyBits;
+    while ((1ull << (bits - kKeyBits)) <= n_views) bits++;  // room for every id and the out-of-range key
+    const size_t b = (size_t)n * sizeof(uint32_t);
+    hipError_t e = hipSuccess;
+    for (uint32_t** p : {&ro.keys[0], &ro.keys[1], &ro.vals[0], &ro.vals[1]}) {
+        if ((e = hipMallocFromPoolAsync(reinterpret_cast<void**>(p), b, pool, s)) != hipSuccess) {
+            release_order(ro, s);
+            return e;
+        }
+    }
+    hipLaunchKernelGGL(k_ray_keys_multi, dim3((n + 255) / 256), dim3(256), 0, s, static_cast<const float4*>(rays),
+                       ray_scene, n, views, n_views, ro.keys[0], ro.vals[0]);
+    if ((e = hipGetLastError()) != hipSuccess) {
+        release_order(ro, s);
+        return e;
+    }
+    hipcub::DoubleBuffer<uint32_t> kb(ro.keys[0], ro.keys[1]), vb(ro.vals[0], ro.vals[1]);
+    size_t tb = 0;
+    // out-of-range ids carry the all-ones key: sorting `bits` bits keeps them last
+    if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (int)n, 0, bits < 32 ? bits : 32, s)) !=
+            hipSuccess ||
+        (e = hipMallocFromPoolAsync(&ro.temp, tb ? tb : 1, pool, s)) != hipSuccess ||
+        (e = hipcub::DeviceRadixSort::SortPairs(ro.temp, tb, kb, vb, (int)n, 0, bits < 32 ? bits : 32, s)) !=
+            hipSuccess) {
         release_order(ro, s);
         return e;
     }

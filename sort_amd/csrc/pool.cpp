@@ -180,7 +180,20 @@ int sortgpu_pool_trace(sortgpu_pool* p, const sortgpu_ray* d_rays, const uint32_
     a.width = p->width;
     a.max_stack = p->max_stack;
     apply_tuning(p->ctx, a);
-    hipError_t e = launch_trace(a, any != 0, false, static_cast<hipStream_t>(stream), p->ctx->num_cus);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // traversal order of a large batch: by scene, then by origin cell in that scene's box (results
+    // do not depend on it; an order that cannot be had only costs speed)
+    RayOrder ro;
+    const uint32_t reorder = p->ctx->tuning.reorder;
+    if (reorder == 2 || (reorder != 1 && n >= kReorderMinRays && n <= 0x7fffffffu)) {
+        hipMemPool_t mp = order_pool(p->ctx);
+        if (mp && order_rays_multi(mp, d_rays, d_ray_scene, n, p->d_views, p->n, s, ro) == hipSuccess)
+            a.order = ro.order;
+        else
+            (void)hipGetLastError();
+    }
+    hipError_t e = launch_trace(a, any != 0, false, s, p->ctx->num_cus);
+    if (ro.order) release_order(ro, s);
     if (e != hipSuccess) return hip_fail(p->ctx, e, "pool trace launch");
     return SORTGPU_OK;
 }

@@ -389,6 +389,23 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve(PathState ps, Queues q) {
     }
 }
 
+// Transparent shadows: every shadow ray's attenuation (r, g, b, surfaces) was written over the
+// first half of its ray record; a black one clears its contribution slot, any other scales it
+// (EvaluateDirect's attenuation factor, integratormethod.cpp:134-142 and 181-185; applied here to
+// the finished contribution instead of first in the product, which moves float rounding only).
+__global__ __launch_bounds__(kPtBlock) void k_resolve_att(PathState ps, Queues q) {
+    const uint32_t n = q.count[2];
+    for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
+        const float4 a = q.sray[2 * (size_t)i];
+        const uint32_t slot = q.ssid[i];
+        float4* c = (slot & 1u) ? ps.c1 : ps.c0;
+        const float4 v = c[slot >> 1];
+        c[slot >> 1] = (a.x == 0.0f && a.y == 0.0f && a.z == 0.0f)
+                           ? make_float4(0.f, 0.f, 0.f, 0.f)
+                           : make_float4(v.x * a.x, v.y * a.y, v.z * a.z, 0.f);
+    }
+}
+
 // ---- finalize ----------------------------------------------------------------------------------
 // One thread per (chunk, pixel): adds the chunk's samples of that pixel, in sample order, to the
 // request's film (a request has at most one chunk per wave, so no two threads share a pixel).
@@ -491,10 +508,25 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
         sa.rays = L.q.sray;
         sa.n = 2 * npaths;
         sa.n_dev = &L.q.count[2];
-        sa.out = L.q.socc;
-        if (L.trees) sa.ray_scene = L.q.ssc;
-        if ((e = launch_trace(sa, true, false, s, L.num_cus)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_resolve, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.ps, L.q);
+        if (L.trees ? L.transps != nullptr : L.transp != nullptr) {
+            // Visibility::GetAttenuation: each walk's result overwrites its own ray's first half
+            // (a lane reads its ray before it writes the result, and no other lane reads it)
+            sa.out = L.q.sray;
+            sa.out_stride = 2;
+            if (L.trees) {
+                sa.ray_scene = L.q.ssc;
+                sa.transps = L.transps;
+            } else {
+                sa.transp = L.transp;
+            }
+            if ((e = launch_attenuation(sa, s, L.num_cus)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_resolve_att, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.ps, L.q);
+        } else {
+            sa.out = L.q.socc;
+            if (L.trees) sa.ray_scene = L.q.ssc;
+            if ((e = launch_trace(sa, true, false, s, L.num_cus)) != hipSuccess) return e;
+            hipLaunchKernelGGL(k_resolve, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.ps, L.q);
+        }
         hipLaunchKernelGGL(k_add_totals, dim3(1), dim3(64), 0, s, L.q.count, cur, L.q.totals);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

@@ -77,6 +77,7 @@ PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_dep
     L.q = sc->ws[k].q;
     L.max_depth = max_depth;
     L.num_cus = acc->ctx->num_cus;
+    L.transp = sc->host.transp;
     return L;
 }
 

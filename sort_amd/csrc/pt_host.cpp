@@ -174,7 +174,7 @@ int prepare_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scen
         return SORTGPU_ERR_ARG;
     }
     for (uint32_t m = 0; m < desc->nmaterials; m++)
-        if (desc->materials[m].type > SORTGPU_MAT_DISNEY) {
+        if (desc->materials[m].type > MAT_TRANSPARENT) {
             set_error(ctx, "scene: material %u has unknown type %u", m, desc->materials[m].type);
             return SORTGPU_ERR_ARG;
         }
@@ -219,6 +219,23 @@ int prepare_scene(sortgpu_ctx* ctx, const sortgpu_accel* acc, const sortgpu_scen
         }
         im.mats.back().type = MAT_LAMBERT;
         im.mats.back().base = v3(1.f, 1.f, 1.f);
+
+        // transparency per primitive id (Material::EvaluateTransparency, material/material.h:159-165):
+        // a Transparent material's attenuation; other materials, and the quads' default white
+        // Lambert, have no transparent node
+        bool any_transparent = false;
+        for (uint32_t m = 0; m < desc->nmaterials; m++) any_transparent |= desc->materials[m].type == MAT_TRANSPARENT;
+        if (any_transparent) {
+            im.transp.assign(4 * ((size_t)ntris + nquads), 0.0f);
+            for (uint32_t t = 0; t < ntris; t++) {
+                const MatRec& mr = im.mats[desc->tri_material ? desc->tri_material[t] : 0];
+                if (mr.type != MAT_TRANSPARENT) continue;
+                im.transp[4 * (size_t)t] = mr.base.x;
+                im.transp[4 * (size_t)t + 1] = mr.base.y;
+                im.transp[4 * (size_t)t + 2] = mr.base.z;
+                im.transp[4 * (size_t)t + 3] = 1.0f;
+            }
+        }
 
         // lights: area quads in quad order, then the environment (Scene::genLightDistribution)
         std::vector<float> power;
@@ -301,6 +318,7 @@ void for_each_array(const SceneImage& im, SceneDev& d, F f) {
     f(im.lights.data(), vbytes(im.lights), reinterpret_cast<const void**>(&d.lights));
     f(im.cdf.data(), vbytes(im.cdf), reinterpret_cast<const void**>(&d.light_cdf));
     f(im.quad_light.data(), vbytes(im.quad_light), reinterpret_cast<const void**>(&d.quad_light));
+    f(im.transp.data(), vbytes(im.transp), reinterpret_cast<const void**>(&d.transp));
 }
 }  // namespace
 

@@ -61,6 +61,7 @@ struct SceneImage {
     std::vector<QuadRec> quads;
     std::vector<uint32_t> idx, tri_mat;
     std::vector<float> pos, nrm, tan, cdf;
+    std::vector<float> transp;          // 4 per primitive id when some material is transparent, else empty
     std::vector<MatRec> mats;
     std::vector<LightRec> lights;
     std::vector<int32_t> quad_light;

@@ -32,6 +32,11 @@ struct PtLaunch {
     uint32_t pix_cap;           // grid bound: at least the wave's film pixels
     int num_cus;
     RayDumpDev dump;            // rays == null: none
+    // single-scene form: the scene's transparency per primitive id (SceneDev::transp), or null;
+    // multi-scene form: a device table of those pointers per tree index, or null. Set, shadow rays
+    // take the transparent-shadow walk and scale their contributions.
+    const float* transp;
+    const float* const* transps;
 };
 
 // Bytes of path workspace per path slot (state + queues), used for HBM budgeting.

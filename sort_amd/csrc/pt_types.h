@@ -24,6 +24,9 @@ struct SceneDev {
     int32_t env_light;         // index of the environment light, or -1
     V3 env_L;
     float clamp;               // > 0: per-sample radiance clamp
+    // Material::EvaluateTransparency per primitive id (r, g, b, has-transparency), or null when no
+    // material is transparent: then shadow rays are any-hit queries, else transparent-shadow walks
+    const float* transp;
 };
 
 struct CamDev {

@@ -268,7 +268,9 @@ enum class MapResult { Mapped, Approx, Unmapped };
 // node, read from its constant inputs. Diffuse (Lambert) -> Lambert (closures.cpp lambert);
 // MicrofacetReflection with GGX -> GGX with conductor Fresnel eta = InteriorIOR, k =
 // AbsorptionCoefficient; Glass -> rough dielectric (dielectric.cpp:32-36: IOR 1.5 inside, 1
-// outside); Disney BRDF -> the Disney subset. A material without a valid surface shader renders
+// outside); Disney BRDF -> the Disney subset; Transparent -> the Transparent BXDF with its
+// attenuation (bsdf/transparent.cpp; the material's transparent node, material.cpp:150, makes its
+// shadow rays take the transparent-shadow walk). A material without a valid surface shader renders
 // as white Lambert in SORT (material.cpp:230-235), so does anything else here (flagged).
 // `sources`: shader unit type -> TSL source (the distribution of a microfacet node and the thin
 // flag of a Disney node exist only there; one source per type, as SORT compiles it).
@@ -289,7 +291,8 @@ MapResult map_material(const FileMaterial& fm, const std::vector<std::pair<std::
                 if (n.name == c.src) bxdf = &n;
         }
     if (!bxdf) return MapResult::Unmapped;
-    bool approx = fm.transparent || fm.sss;
+    // a transparent node anywhere but as the output's own BXDF (a mix, an add) is not carried
+    bool approx = (fm.transparent && bxdf->type != "SORTNode_Material_Transparent") || fm.sss;
     for (const Conn& c : fm.surface.conns)
         if (c.dst == bxdf->name && c.dst_prop != "Normal") approx = true;  // a node drives an input
     auto f1 = [&](const char* name, float dflt) {
@@ -339,6 +342,9 @@ MapResult map_material(const FileMaterial& fm, const std::vector<std::pair<std::
             f1("DiffuseTransmittance", 0.0f) != 0.0f || sd[0] > 0.0f || sd[1] > 0.0f || sd[2] > 0.0f)
             approx = true;
         if (!src || src->find(", true ,") != std::string::npos) approx = true;  // thin surface
+    } else if (t == "SORTNode_Material_Transparent") {
+        out.type = SORTGPU_MAT_TRANSPARENT;
+        f3("Attenuation", out.base, 1.0f);
     } else {
         return MapResult::Unmapped;
     }

@@ -74,6 +74,9 @@ class HipDevice : public Device {
         d_scenes_ = static_cast<SceneDev*>(p);
         if ((e = hipMalloc(&p, tb)) != hipSuccess) return (err = hip_msg("hipMalloc(tree table)", e), hip_rc(e));
         d_trees_ = static_cast<TreeView*>(p);
+        const size_t pb = (size_t)max_scenes_ * sizeof(const float*);
+        if ((e = hipMalloc(&p, pb)) != hipSuccess) return (err = hip_msg("hipMalloc(transparency table)", e), hip_rc(e));
+        d_transps_ = static_cast<const float**>(p);
         if ((e = hipMalloc(&p, cb + 256)) != hipSuccess) return (err = hip_msg("hipMalloc(counters)", e), hip_rc(e));
         d_counters_ = static_cast<uint32_t*>(p);
         d_error_ = d_counters_ + cb / sizeof(uint32_t);
@@ -83,17 +86,22 @@ class HipDevice : public Device {
         if ((e = hipHostMalloc(&p, tb, hipHostMallocDefault)) != hipSuccess)
             return (err = hip_msg("hipHostMalloc(tree table)", e), hip_rc(e));
         h_trees_ = static_cast<TreeView*>(p);
+        if ((e = hipHostMalloc(&p, pb, hipHostMallocDefault)) != hipSuccess)
+            return (err = hip_msg("hipHostMalloc(transparency table)", e), hip_rc(e));
+        h_transps_ = static_cast<const float**>(p);
         std::memset(h_scenes_, 0, sb);
         std::memset(h_trees_, 0, tb);
+        std::memset(h_transps_, 0, pb);
         if ((e = hipMemcpy(d_scenes_, h_scenes_, sb, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(d_trees_, h_trees_, tb, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(d_transps_, h_transps_, pb, hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemset(d_counters_, 0, cb + 256)) != hipSuccess)
             return (err = hip_msg("table initialisation", e), hip_rc(e));
         for (uint32_t i = 0; i < ns_; i++)
             if ((e = pt_zero_f4(reinterpret_cast<float4*>(ws_[i].q.totals), 1, s_[i])) != hipSuccess ||
                 (e = hipStreamSynchronize(s_[i])) != hipSuccess)
                 return (err = hip_msg("workspace initialisation", e), hip_rc(e));
-        fixed_ = sb + tb + cb + 256 + (size_t)ReqRing::kSlots * sizeof(WaveDesc);
+        fixed_ = sb + tb + pb + cb + 256 + (size_t)ReqRing::kSlots * sizeof(WaveDesc);
         for (uint32_t i = 0; i < ns_; i++) fixed_ += ws_[i].mem.bytes + sizeof(WaveDesc);
         return SORTGPU_OK;
     }
@@ -116,6 +124,8 @@ class HipDevice : public Device {
         if (d_counters_) hipFree(d_counters_);
         if (h_scenes_) hipHostFree(h_scenes_);
         if (h_trees_) hipHostFree(h_trees_);
+        if (d_transps_) hipFree(d_transps_);
+        if (h_transps_) hipHostFree(h_transps_);
         if (ev_up_) hipEventDestroy(ev_up_);
         if (ev_film_) hipEventDestroy(ev_film_);
         for (uint32_t i = 0; i < ns_; i++)
@@ -166,10 +176,13 @@ class HipDevice : public Device {
         }
         h_scenes_[id] = d;
         h_trees_[id] = d.tree;
+        h_transps_[id] = d.transp;
         if ((e = hipMemcpyAsync(d_scenes_ + id, h_scenes_ + id, sizeof(SceneDev), hipMemcpyHostToDevice, s_up_)) !=
                 hipSuccess ||
             (e = hipMemcpyAsync(d_trees_ + id, h_trees_ + id, sizeof(TreeView), hipMemcpyHostToDevice, s_up_)) !=
                 hipSuccess ||
+            (e = hipMemcpyAsync(d_transps_ + id, h_transps_ + id, sizeof(const float*), hipMemcpyHostToDevice,
+                                s_up_)) != hipSuccess ||
             (e = hipEventRecord(ev_up_, s_up_)) != hipSuccess)
             return (err = hip_msg("scene table update", e), hip_rc(e));
         return SORTGPU_OK;
@@ -263,12 +276,16 @@ class HipDevice : public Device {
         int rc = SORTGPU_OK;
         if (e == hipSuccess) rc = ring_.push(ctx_, wd, s, d_desc_[k], &d_wave);
         if (e == hipSuccess && rc == SORTGPU_OK) {
-            PtLaunch L = wave_launch(k, w.max_depth);
+            // a wave holding a scene with transparent materials answers its shadow rays with the
+            // transparent-shadow walk (opaque scenes' rays in it get the same results as any-hit)
+            bool att = false;
+            for (const Chunk& c : w.chunks) att |= !scenes_[c.req->p.scene].im.transp.empty();
+            PtLaunch L = wave_launch(k, w.max_depth, att);
             L.paths_cap = ct.total_paths;
             L.pix_cap = ct.total_pix;
             if (graphs_) {
                 hipGraphExec_t x = nullptr;
-                if ((rc = wave_graph(k, w.max_depth, &x)) == SORTGPU_OK) e = hipGraphLaunch(x, s);
+                if ((rc = wave_graph(k, w.max_depth, att, &x)) == SORTGPU_OK) e = hipGraphLaunch(x, s);
             } else {
                 e = pt_wave_paths(L, s);
             }
@@ -361,7 +378,7 @@ class HipDevice : public Device {
 
     // Launch parameters of the waves of stream k: its workspace, its descriptor and its own work
     // counter (waves of one stream run one after another).
-    PtLaunch wave_launch(uint32_t k, uint32_t depth) const {
+    PtLaunch wave_launch(uint32_t k, uint32_t depth, bool att) const {
         PtLaunch L;
         std::memset(&L, 0, sizeof(L));
         L.trace.counter = d_counters_ + k * kCounterStride;
@@ -377,6 +394,7 @@ class HipDevice : public Device {
         L.q = ws_[k].q;
         L.max_depth = depth;
         L.num_cus = ctx_->num_cus;
+        L.transps = att ? d_transps_ : nullptr;
         return L;
     }
 
@@ -384,7 +402,7 @@ class HipDevice : public Device {
     // sized for the workspace's capacity; the kernels read the wave's exact counts from its
     // descriptor). Captured on first use per depth, again when a scene changes the tree table's
     // width or stack depth or the context's tuning changes. The graph holds kernel launches only.
-    int wave_graph(uint32_t k, uint32_t depth, hipGraphExec_t* out) {
+    int wave_graph(uint32_t k, uint32_t depth, bool att, hipGraphExec_t* out) {
         GraphKey key;
         std::memset(&key, 0, sizeof(key));
         key.tuning = ctx_->tuning;
@@ -394,13 +412,14 @@ class HipDevice : public Device {
             drop_graphs();
             graph_key_ = key;
         }
-        auto it = gexec_[k].find(depth);
+        const uint32_t gk = 2 * depth + (att ? 1u : 0u);
+        auto it = gexec_[k].find(gk);
         if (it != gexec_[k].end()) {
             *out = it->second;
             return SORTGPU_OK;
         }
         hipStream_t s = s_[k];
-        PtLaunch L = wave_launch(k, depth);
+        PtLaunch L = wave_launch(k, depth, att);
         L.paths_cap = cap_;
         L.pix_cap = cap_;
         hipGraph_t g = nullptr;
@@ -416,7 +435,7 @@ class HipDevice : public Device {
         e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) return hip_fail(ctx_, e, "hipGraphInstantiate(wave)");
-        gexec_[k][depth] = x;
+        gexec_[k][gk] = x;
         *out = x;
         return SORTGPU_OK;
     }
@@ -451,7 +470,7 @@ class HipDevice : public Device {
     uint32_t cap_, max_scenes_, ns_;
     bool graphs_;
     WaveDesc* d_desc_[kMaxStreams] = {};
-    std::map<uint32_t, hipGraphExec_t> gexec_[kMaxStreams];
+    std::map<uint32_t, hipGraphExec_t> gexec_[kMaxStreams];  // key: 2 * depth + transparent-shadow form
     struct GraphKey {  // what a captured wave graph bakes in besides the depth
         sortgpu_tuning tuning;
         uint32_t width, max_stack;
@@ -467,6 +486,8 @@ class HipDevice : public Device {
     TreeView* d_trees_ = nullptr;
     SceneDev* h_scenes_ = nullptr;
     TreeView* h_trees_ = nullptr;
+    const float** d_transps_ = nullptr;  // per scene: its transparency table on the device, or null
+    const float** h_transps_ = nullptr;
     uint32_t* d_counters_ = nullptr;
     uint32_t* d_error_ = nullptr;
     uint32_t slot_ = 0;

@@ -216,7 +216,7 @@ __device__ __forceinline__ V3 fresnel_conductor(float cosI, V3 eta, V3 k) {
 
 // ---------------------------------------------------------------------------------------------
 // Materials
-enum MatType : uint32_t { MAT_LAMBERT = 0, MAT_GGX = 1, MAT_DIELECTRIC = 2, MAT_DISNEY = 3 };
+enum MatType : uint32_t { MAT_LAMBERT = 0, MAT_GGX = 1, MAT_DIELECTRIC = 2, MAT_DISNEY = 3, MAT_TRANSPARENT = 4 };
 
 struct alignas(16) MatRec {
     uint32_t type;
@@ -506,6 +506,7 @@ __device__ __forceinline__ V3 mat_f(const MatRec& m, V3 wo, V3 wi) {
             dielectric_parts(m, r, t);
             return mft_f(t, wo, wi) + mfr_f(r, wo, wi);
         }
+        case MAT_TRANSPARENT: return v3(0.f, 0.f, 0.f);  // Transparent::f (bsdf/transparent.cpp:24-26)
         default: return disney_f(m, wo, wi);
     }
 }
@@ -519,6 +520,7 @@ __device__ __forceinline__ float mat_pdf(const MatRec& m, V3 wo, V3 wi) {
             return mfr_pdf(s, wo, wi);
         }
         case MAT_DIELECTRIC: return dielectric_pdf(m, wo, wi);
+        case MAT_TRANSPARENT: return 0.0f;  // Transparent::pdf (bsdf/transparent.cpp:28-30)
         default: return disney_pdf(m, wo, wi);
     }
 }
@@ -551,6 +553,13 @@ __device__ __forceinline__ BsdfSampleOut mat_sample(const MatRec& m, V3 wo, floa
             if (bs_u < ratio || ratio == 1.0f) o = mfr_sample(r, wo, fr.b, fr.c);
             else o = mft_sample(t, wo, fr.b, fr.c);
             o.pdf = lerp(mft_pdf(t, wo, o.wi), mfr_pdf(r, wo, o.wi), ratio);
+            return o;
+        }
+        case MAT_TRANSPARENT: {  // Transparent::sample_f (bsdf/transparent.cpp:32-37): straight through
+            BsdfSampleOut o;
+            o.wi = -wo;
+            o.pdf = 1.0f;
+            o.f = m.base;  // the attenuation A
             return o;
         }
         default: return disney_sample(m, wo, fr.a, fr.b, fr.c);

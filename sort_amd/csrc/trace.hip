@@ -79,11 +79,11 @@ __device__ __forceinline__ void leaf_step(const RayK& r, const PrimRec* __restri
     bool blocked = false;
     if (TRANSP) {
         if (pr.flags & PR_QUAD) {
-            blocked = hit && transp[4 * (size_t)bp + 3] == 0.0f;
+            blocked = hit && (!transp || transp[4 * (size_t)bp + 3] == 0.0f);
         } else {
             gup |= hit;
             if (pr.flags & PR_GEND) {
-                blocked = gup && transp[4 * (size_t)bp + 3] == 0.0f;
+                blocked = gup && (!transp || transp[4 * (size_t)bp + 3] == 0.0f);
                 gup = false;
             }
         }
@@ -115,6 +115,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const PrimRec* __restrict__ prims = a.tree.prims;
     const QuadRec* __restrict__ quads = a.tree.quads;
     const float* __restrict__ transp = a.transp;
+    const uint32_t ostride = a.out_stride ? a.out_stride : 1u;  // attenuation output stride (float4s)
+    uint32_t scene = 0;  // multi-scene transparent walk: the lane's tree index (restarts reread its view)
 
     RayK r;
     uint32_t li = 0xffffffffu;  // next primitive record of this lane's leaf
@@ -192,6 +194,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                             nodes = static_cast<const NodeRec<W>*>(tv.nodes);
                             prims = tv.prims;
                             quads = tv.quads;
+                            if (TRANSP) {
+                                scene = sid;
+                                transp = a.transps[sid];
+                            }
                             root = tv.root;
                             valid = tv.valid;
                             if (valid) rmin = root_slab(r, tv.bbox);
@@ -211,7 +217,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                         push(0, root, rmin);
                         sp = 1;
                     } else {
-                        if (TRANSP) reinterpret_cast<float4*>(a.out)[my] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                        if (TRANSP) reinterpret_cast<float4*>(a.out)[(size_t)my * ostride] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
                         else if (ANY) static_cast<uint8_t*>(a.out)[my] = 0;
                         else reinterpret_cast<float4*>(a.out)[my] =
                                  make_float4(FLT_MAX, __uint_as_float(0xffffffffu), 0.f, 0.f);
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             if (occluded) {
                 att_r = att_g = att_b = 0.0f;
             } else if (bp != 0xffffffffu) {
-                const float4 tr = reinterpret_cast<const float4*>(transp)[bp];
+                const float4 tr = transp ? reinterpret_cast<const float4*>(transp)[bp] : make_float4(0.f, 0.f, 0.f, 0.f);
                 const bool has = tr.w != 0.0f;
                 const float ar = has ? tr.x : 0.0f, ag = has ? tr.y : 0.0f, ab = has ? tr.z : 0.0f;
                 if (ar == 0.0f && ag == 0.0f && ab == 0.0f) {
@@ -337,10 +343,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                             r.tmin = 0.001f;
                             r.tmax = r.tmax - bt;
                             ray_setup(r);
-                            r.ieee = ieee_slabs_ok(r, a.tree.bbox);
-                            const float rmin = a.tree.valid ? root_slab(r, a.tree.bbox) : -1.0f;
+                            const TreeView& tv = MULTI ? a.trees[scene] : a.tree;
+                            r.ieee = ieee_slabs_ok(r, tv.bbox);
+                            const float rmin = tv.valid ? root_slab(r, tv.bbox) : -1.0f;
                             if (rmin >= 0.0f) {
-                                push(0, a.tree.root, rmin);
+                                push(0, tv.root, rmin);
                                 sp = 1;
                                 bt = FLT_MAX;
                                 bp = 0xffffffffu;
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 }
             }
             if (done) {
-                reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(att_r, att_g, att_b, (float)steps);
+                reinterpret_cast<float4*>(a.out)[(size_t)ray_id * ostride] = make_float4(att_r, att_g, att_b, (float)steps);
                 ray_id = 0xffffffffu;
             }
         }
@@ -444,18 +451,22 @@ hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t st
     return multi ? launch_w<4, true>(a, any, stats, stream, num_cus) : launch_w<4, false>(a, any, stats, stream, num_cus);
 }
 
-template <int W>
+template <int W, bool MULTI>
 static hipError_t launch_att_w(const TraceArgs& a, hipStream_t stream, int num_cus) {
-    const uint32_t need = a.tree.max_stack;
-    if (need <= 16u + 48u) return launch_one<W, false, false, 48, false, 16, true>(a, stream, num_cus);
-    if (need <= 16u + 240u) return launch_one<W, false, false, 240, false, 16, true>(a, stream, num_cus);
+    const uint32_t need = MULTI ? a.max_stack : a.tree.max_stack;
+    if (need <= 16u + 48u) return launch_one<W, false, false, 48, MULTI, 16, true>(a, stream, num_cus);
+    if (need <= 16u + 240u) return launch_one<W, false, false, 240, MULTI, 16, true>(a, stream, num_cus);
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_attenuation(const TraceArgs& a, hipStream_t stream, int num_cus) {
     if (a.n == 0) return hipSuccess;
-    if (!a.transp || a.trees != nullptr) return hipErrorInvalidValue;
-    return a.tree.width == 8 ? launch_att_w<8>(a, stream, num_cus) : launch_att_w<4>(a, stream, num_cus);
+    if (a.trees) {
+        if (!a.transps) return hipErrorInvalidValue;
+        return a.width == 8 ? launch_att_w<8, true>(a, stream, num_cus) : launch_att_w<4, true>(a, stream, num_cus);
+    }
+    if (!a.transp) return hipErrorInvalidValue;
+    return a.tree.width == 8 ? launch_att_w<8, false>(a, stream, num_cus) : launch_att_w<4, false>(a, stream, num_cus);
 }
 
 uint32_t trace_max_stack_supported() { return 8 + 240; }

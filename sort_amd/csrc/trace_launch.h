@@ -32,14 +32,20 @@ struct TraceArgs {
     // permutation of 0..n-1, device), its result goes to out[order[k]]; null = index order.
     const uint32_t* order = nullptr;
     // transparent-shadow attenuation only: per primitive id, 4 floats (transparency r, g, b, and
-    // 1 when the primitive's material has transparency, else 0)
+    // 1 when the primitive's material has transparency, else 0); in the multi-scene form, a device
+    // table of such pointers per tree index instead (a null entry: every primitive opaque)
     const float* transp = nullptr;
+    const float* const* transps = nullptr;
+    // transparent-shadow attenuation only: ray i's result goes to float4 index i * out_stride of out
+    // (0 or 1: packed; 2: over the first half of ray i's own 32-byte record when out == rays)
+    uint32_t out_stride = 0;
 };
 
 // Launches the closest (any = false) or any-hit traversal on `stream`.
 hipError_t launch_trace(const TraceArgs& a, bool any, bool stats, hipStream_t stream, int num_cus);
-// Launches the transparent-shadow attenuation walk (single tree, a.transp set): out receives one
-// float4 per ray (attenuation r, g, b, and the number of surfaces passed as a float).
+// Launches the transparent-shadow attenuation walk (single tree with a.transp, or the multi-scene
+// form with a.trees and a.transps): out receives one float4 per ray (attenuation r, g, b, and the
+// number of surfaces passed as a float), at stride a.out_stride.
 hipError_t launch_attenuation(const TraceArgs& a, hipStream_t stream, int num_cus);
 // Most surfaces one attenuation walk passes before it gives up (error bit 2).
 constexpr uint32_t kMaxAttenuationSteps = 4096;
@@ -70,6 +76,9 @@ struct RayOrder {
 hipError_t order_rays(hipMemPool_t pool, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
                       RayOrder& ro);
 void release_order(RayOrder& ro, hipStream_t s);
+// The same for rays of many trees: sorted by (tree index, origin cell in that tree's box).
+hipError_t order_rays_multi(hipMemPool_t pool, const void* rays, const uint32_t* ray_scene, uint32_t n,
+                            const TreeView* views, uint32_t n_views, hipStream_t s, RayOrder& ro);
 // Traversal order without allocation (the no-allocation entry point): buffers for batches of up
 // to kScratchRays rays live in a file-scope device array of kScratchBytes per device.
 constexpr uint32_t kScratchRays = 1u << 22;

@@ -8,7 +8,8 @@ wavefront on the GPU.
 
 Materials are plain parameter records (SORT builds the same BXDFs from TSL closures):
 ``lambert(rgb)``, ``ggx(rgb, roughness, eta, k)``, ``dielectric(reflectance, transmittance,
-roughness, ior)``, ``disney(base, ...)``.
+roughness, ior)``, ``disney(base, ...)``, ``transparent(attenuation)``. A scene with a transparent
+material answers its shadow rays with SORT's transparent-shadow walk (Scene::GetAttenuation).
 """
 from __future__ import annotations
 
@@ -20,7 +21,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 
-LAMBERT, GGX, DIELECTRIC, DISNEY = 0, 1, 2, 3
+LAMBERT, GGX, DIELECTRIC, DISNEY, TRANSPARENT = 0, 1, 2, 3, 4
 
 
 def _mat(type_, base=(1, 1, 1), trans=(0, 0, 0), roughness=0.5, ior=1.5, eta=(1, 1, 1), k=(0, 0, 0), metallic=0.0,
@@ -48,6 +49,12 @@ def disney(base, metallic=0.0, specular=0.5, specular_tint=0.0, roughness=0.5, s
     return _mat(DISNEY, base=base, metallic=metallic, specular=specular, specular_tint=specular_tint,
                 roughness=roughness, sheen=sheen, sheen_tint=sheen_tint, clearcoat=clearcoat,
                 clearcoat_gloss=clearcoat_gloss)
+
+
+def transparent(attenuation):
+    """SORT's Transparent BXDF (the Transparent shader node, bsdf/transparent.cpp): light passes
+    straight through, scaled by `attenuation`, both along paths and along shadow rays."""
+    return _mat(TRANSPARENT, base=attenuation)
 
 
 def material_array(mats):

@@ -89,3 +89,27 @@ def test_tunings_and_widths_give_identical_hits(gpu_ctx, work8):
     got4 = pool4.trace(d_rays, d_sid).cpu().numpy()
     same = got4[:, 0].view(np.uint32) == ref[:, 0].view(np.uint32)
     assert same.mean() >= 1 - 1e-5, (~same).sum()
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_pool_traversal_order_changes_no_result(gpu_ctx, work8, any_hit):
+    """A pooled batch taken in (scene, origin-cell) order gives every ray the same result as in
+    index order, including rays whose scene id is outside the pool (they sort last and miss)."""
+    w = work8
+    pool = batch.Pool(gpu_ctx, w["accels"])
+    d_rays = torch.from_numpy(w["rays"]).cuda()
+    sid = w["scene_of_ray"].copy()
+    sid[::101] = 1 << 20  # out of range
+    d_sid = torch.from_numpy(sid.view(np.int32)).cuda()
+    print(f"[batch] order never / always over {len(sid)} rays, any={any_hit}", flush=True)
+    try:
+        batch.set_tuning(gpu_ctx, reorder=1)
+        plain = pool.trace(d_rays, d_sid, any_hit=any_hit).cpu().numpy()
+        batch.set_tuning(gpu_ctx, reorder=2)
+        ordered = pool.trace(d_rays, d_sid, any_hit=any_hit).cpu().numpy()
+    finally:
+        batch.set_tuning(gpu_ctx)
+    assert np.array_equal(plain.view(np.uint8), ordered.view(np.uint8))
+    with pytest.raises(Exception, match="outside the pool"):
+        pool.check_errors()
+    pool.close()

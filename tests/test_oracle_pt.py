@@ -272,3 +272,41 @@ def test_per_sample_clamp_closed_forms():
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     c = O.OraclePT(dict(sc, clamp=0.5)).render(room, 24, 24, 4, 5, 9, threads=8)
     assert np.all(c <= np.float32(0.5)) and np.all(c >= 0) and np.any(a > 0.5)
+
+
+def test_transparent_sheet_attenuates_direct_light_closed_form():
+    """SORT's transparent shadows (ENABLE_TRANSPARENT_SHADOW, integratormethod.cpp:121-145 and
+    168-186; Scene::GetAttenuation, core/scene.cpp:192-208) and its Transparent BXDF
+    (bsdf/transparent.cpp): a sheet with attenuation A between a Lambert ground and a quad light
+    scales the ground's direct light by A in both MIS halves; seen through the sheet (the camera
+    ray crosses it first, continuing straight with weight A), the ground's direct light is scaled
+    by A twice. Expected values from Lambert's polygon irradiance formula."""
+    rho, L = 0.5, 10.0
+    A = np.array([0.5, 0.25, 0.8])
+    ground = [np.array([[-10, 0, -10], [10, 0, 10], [10, 0, -10]], np.float32),
+              np.array([[-10, 0, -10], [-10, 0, 10], [10, 0, 10]], np.float32)]
+    sheet = [np.array([[-10, 0.7, -10], [10, 0.7, 10], [10, 0.7, -10]], np.float32),
+             np.array([[-10, 0.7, -10], [-10, 0.7, 10], [10, 0.7, 10]], np.float32)]
+    pos, idx = scenes.tris_to_mesh(ground + sheet)
+    nrm = np.tile(np.array([[0, 1, 0]], np.float32), (len(pos), 1))
+    light = scenes.translate_quad((0.0, 1.0, 0.0), 1.0, facing_down=True)
+    sc = {"pos": pos, "idx": idx, "quads": [light], "normals": nrm,
+          "tri_material": np.array([0, 0, 1, 1], np.uint32),
+          "materials": [I.lambert((rho, rho, rho)), I.transparent(tuple(A))],
+          "quad_radiance": np.array([[L, L, L]], np.float32), "env": (0, 0, 0)}
+    pt = O.OraclePT(sc)
+    pts = [(0.0, 0.0), (0.3, 0.1), (0.7, -0.4), (1.5, 0.0)]
+    spp = 1 << 14
+    corners = [np.array(c) for c in [(-0.5, 1, -0.5), (0.5, 1, -0.5), (0.5, 1, 0.5), (-0.5, 1, 0.5)]]
+    for y0, depth, power in ((0.5, 1, 1), (0.9, 2, 2)):  # below the sheet; above it, looking through
+        rays, keys = [], []
+        for i, (x, z) in enumerate(pts):
+            for k in range(spp):
+                rays.append([x, y0, z, 0.0, 0.0, -1.0, 0.0, 3.4028234663852886e38])
+                keys.append([i, k])
+        out = pt.li(np.array(rays, np.float32), np.array(keys, np.uint32), max_depth=depth, seed=13, threads=8)
+        est = out.reshape(len(pts), spp, 3).mean(axis=1)
+        for (x, z), e in zip(pts, est):
+            E = _polygon_irradiance(np.array([x, 0.0, z]), np.array([0, 1, 0]), corners, L)
+            want = A ** power * rho / math.pi * E
+            assert np.all(np.abs(e - want) <= 0.01 * want + 1e-4), (y0, x, z, e, want)

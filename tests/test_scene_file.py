@@ -141,6 +141,8 @@ def test_materials_map_to_gpu_records():
         {"name": "textured", "kind": O.DIFFUSE, "texture_input": True},          # a node drives Diffuse
         {"name": "aniso", "kind": O.GGX_NODE, "params": {"RoughnessU": 0.1, "RoughnessV": 0.4}},
         {"name": "none", "kind": None},                                           # invalid surface shader
+        {"name": "tint", "kind": O.TRANSPARENT_NODE, "params": {"Attenuation": (0.5, 0.25, 0.8)}},
+        {"name": "lam_tr", "kind": O.DIFFUSE, "transparent": True},              # transparency elsewhere in a graph
     ]
     verts, tris = _quad_mesh()
     tris[0, 3], tris[1, 3] = -1, 42  # -1 and an unknown id: the default material
@@ -148,16 +150,17 @@ def test_materials_map_to_gpu_records():
     o, g = O.read_scene(data), scene_file.parse(data)
     _assert_equal_scenes(g, o)
     m = g["materials"]
-    assert [x["type"] for x in m] == [0, 1, 2, 3, 0, 0, 1, 0, 0]
+    assert [x["type"] for x in m] == [0, 1, 2, 3, 0, 0, 1, 0, 4, 0, 0]
+    assert m[8]["base"] == pytest.approx((0.5, 0.25, 0.8))
     assert m[0]["base"] == pytest.approx((0.1, 0.2, 0.3))
     assert m[1]["eta"] == pytest.approx((0.2, 0.9, 1.1)) and m[1]["k"] == pytest.approx((3.9, 2.4, 2.1))
     assert m[1]["roughness"] == pytest.approx(0.3)
     assert m[2]["ior"] == 1.5 and m[2]["trans"] == pytest.approx((0.9, 0.95, 1.0))
     assert m[3]["metallic"] == pytest.approx(0.25) and m[3]["clearcoat"] == pytest.approx(0.5)
-    assert m[4]["base"] == (1.0, 1.0, 1.0) and m[7]["base"] == (1.0, 1.0, 1.0) and m[8]["base"] == (1.0, 1.0, 1.0)
+    assert m[4]["base"] == (1.0, 1.0, 1.0) and m[7]["base"] == (1.0, 1.0, 1.0) and m[10]["base"] == (1.0, 1.0, 1.0)
     assert g["material_names"] == [x["name"] for x in mats] + ["default"]
-    assert g["tri_material"].tolist() == [8, 8]
-    assert g["skipped"] == {"unmapped_materials": 1, "approximated_materials": 2}
+    assert g["tri_material"].tolist() == [10, 10]
+    assert g["skipped"] == {"unmapped_materials": 1, "approximated_materials": 3}
 
 
 def _random_scene(seed):
