@@ -164,8 +164,30 @@ def leg_batched(ctx):
     res = {"scenes": 64, "tris": int(w["tris"].sum()), "rays": n, "pool_mrays_s": round(n / ms_pool / 1e3, 1),
            "per_scene_mrays_s": round(n / ms_each / 1e3, 1), "speedup": round(ms_each / ms_pool, 2),
            "pool_bytes": pool.info()["device_bytes"], "build_s": round(build_s, 1)}
+    # the autotuner over node width (the same scenes as 4-wide trees), LDS stack depth, compaction
+    # threshold (refill), resident workgroups per CU and leaf batch, timed on the whole workload
+    from sort_amd import Qbvh
+    t = time.perf_counter()
+    pool4 = batch.Pool(ctx, [Qbvh(ctx).build(pos, idx) for pos, idx in w["geometry"]])
+    build4_s = time.perf_counter() - t
+
+    def runner(p):
+        return lambda: p.trace(d_rays, d_sid, out=out)
+
+    pools = {8: pool, 4: pool4}
+    t = time.perf_counter()
+    best, results = autotune.tune(ctx, {8: runner(pool), 4: runner(pool4)}, lds=(16, 8), refills=(8, 16, 32),
+                                  blocks=(0, 2), leaf_batches=(16, 32), reps=3)
+    tune_s = time.perf_counter() - t
+    ms_tuned = autotune._time(runner(pools[best["width"]]), reps=5, warmup=2)
+    batch.set_tuning(ctx)  # defaults for the legs that follow
+    res["autotune"] = {"best": {k: best[k] for k in ("width", "lds_entries", "refill", "blocks_per_cu", "leaf_batch")},
+                       "candidates": len(results), "tune_s": round(tune_s, 1), "build_4wide_s": round(build4_s, 1),
+                       "tuned_mrays_s": round(n / ms_tuned / 1e3, 1),
+                       "tuned_over_default": round(ms_pool / ms_tuned, 3)}
     log(f"[batched] {res}")
     pool.close()
+    pool4.close()
     return res
 
 
