@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--tris", type=int, default=1 << 20)
     p.add_argument("--legs", default="intersection,attenuation,multi,film,pt,batched,served")
     p.add_argument("--served-requests", type=int, default=60)
+    p.add_argument("--shade-order", type=int, default=0, choices=[0, 1, 2],
+                   help="path tracer: shade each bounce's hits in spatial order (0 default, 1 never, 2 always)")
     p.add_argument("--copy-probe", action="store_true",
                    help="end with three 256 MiB device copies (HBM counter calibration)")
     return p.parse_args()
@@ -534,10 +536,13 @@ def main():
     del d_inc, d_sh, d_cam, hits, occ
     acc.release()
     torch.cuda.empty_cache()
+    from sort_amd.batch import set_tuning
+    set_tuning(ctx, shade_order=a.shade_order)
     if "pt" in legs:
         out["pt"] = leg_pt(ctx, cpu=not a.no_cpu)
     if "batched" in legs:
         out["batched"] = leg_batched(ctx)
+        set_tuning(ctx, shade_order=a.shade_order)  # the batched leg leaves the defaults applied
     if "served" in legs:
         from sort_amd.serve import bench_served
         out["served"] = bench_served(ctx, n_requests=a.served_requests, log=log)

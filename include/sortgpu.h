@@ -227,8 +227,11 @@ int sortgpu_accel_errors(sortgpu_accel* accel, void* stream, uint32_t* flags);
  * (64^3 grid over the scene box; pooled rays by scene, then by cell in their scene's box) before the
  * launch (stream-ordered device memory of 16 bytes per ray plus sort workspace, from a pool the
  * context owns; a batch sharing one origin, such as camera rays, keeps its order and pays only the
- * sort), 1 = never, 2 = always. The knobs change speed only: every hit is the same for every
- * setting. */
+ * sort), 1 = never, 2 = always; shade_order = order in which the path tracer shades each bounce's
+ * hits: 0 (default) sorts them by position (a 16^3 grid over the scene box; by scene and an 8^3 grid
+ * in multi-request waves) when the wave holds at least 262,144 paths, so shading, the shadow rays and
+ * the next bounce's rays run in spatial order, 1 = never, 2 = always. The knobs change speed only:
+ * every hit and every image is the same for every setting. */
 typedef struct sortgpu_tuning {
     uint32_t refill;
     uint32_t lds_entries;
@@ -236,7 +239,8 @@ typedef struct sortgpu_tuning {
     uint32_t leaf_batch;
     uint32_t leaf_min;
     uint32_t reorder;
-    uint32_t reserved[2];
+    uint32_t shade_order;
+    uint32_t reserved;
 } sortgpu_tuning;
 
 /* Applies to every later query, render and pool trace of the context. */

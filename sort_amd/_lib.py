@@ -51,7 +51,7 @@ class BuildOpts(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("refill", ctypes.c_uint32), ("lds_entries", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("leaf_batch", ctypes.c_uint32), ("leaf_min", ctypes.c_uint32),
-                ("reorder", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("reorder", ctypes.c_uint32), ("shade_order", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Material(ctypes.Structure):

@@ -66,10 +66,13 @@ class Pool:
             pass
 
 
-def set_tuning(ctx: Context, refill=0, lds_entries=0, blocks_per_cu=0, leaf_batch=0, leaf_min=0, reorder=0):
+def set_tuning(ctx: Context, refill=0, lds_entries=0, blocks_per_cu=0, leaf_batch=0, leaf_min=0, reorder=0,
+               shade_order=0):
     """Traversal knobs of the context (0 = default); they change speed only, never a hit.
-    reorder: 0 = sort large trace batches into coherence order (default), 1 = never, 2 = always."""
-    t = _lib.Tuning(refill, lds_entries, blocks_per_cu, leaf_batch, leaf_min, reorder)
+    reorder: 0 = sort large trace batches into coherence order (default), 1 = never, 2 = always.
+    shade_order: 0 = shade each bounce's hits of waves of 2^18+ paths in spatial order (default),
+    1 = never, 2 = always."""
+    t = _lib.Tuning(refill, lds_entries, blocks_per_cu, leaf_batch, leaf_min, reorder, shade_order)
     check(lib().sortgpu_ctx_set_tuning(ctx.handle, ctypes.byref(t)), ctx.handle)
 
 

@@ -143,10 +143,10 @@ int sortgpu_ctx_set_tuning(sortgpu_ctx* ctx, const sortgpu_tuning* t) {
         return SORTGPU_ERR_ARG;
     }
     if (t->refill > 64 || (t->lds_entries != 0 && t->lds_entries != 8 && t->lds_entries != 16) || t->leaf_batch > 64 ||
-        t->leaf_min > 64 || t->reorder > 2) {
+        t->leaf_min > 64 || t->reorder > 2 || t->shade_order > 2) {
         set_error(ctx,
-                  "sortgpu_ctx_set_tuning: refill, leaf_batch and leaf_min must be <= 64, lds_entries 0, 8 or 16 and "
-                  "reorder 0, 1 or 2");
+                  "sortgpu_ctx_set_tuning: refill, leaf_batch and leaf_min must be <= 64, lds_entries 0, 8 or 16, "
+                  "reorder and shade_order 0, 1 or 2");
         return SORTGPU_ERR_ARG;
     }
     ctx->tuning = *t;

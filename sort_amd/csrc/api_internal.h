@@ -68,6 +68,11 @@ void free_accel_device(sortgpu_accel* acc);
 // Batches at least this large are traced in coherence order by default (smaller ones do not pay
 // for the sort).
 constexpr uint32_t kReorderMinRays = 1u << 18;
+// Waves of at least this many paths shade each bounce's hits in spatial order (tuning shade_order:
+// 0 = this default, 1 = never (0 returned), 2 = always).
+inline uint32_t shade_sort_min(const sortgpu_tuning& t) {
+    return t.shade_order == 1 ? 0u : t.shade_order == 2 ? 1u : (1u << 18);
+}
 // The context's stream-ordered pool for traversal-order buffers (made on first use), or null.
 hipMemPool_t order_pool(sortgpu_ctx* ctx);
 // Takes (and clears) the error flags at d_flags after the work queued on `s`; returns SORTGPU_OK

@@ -211,14 +211,125 @@ __device__ __forceinline__ V3 to_world(const Frame& f, V3 v) {
               v.x * f.t.z + v.y * f.n.z + v.z * f.b.z);
 }
 
+// ---- spatial order of a bounce's hits ----------------------------------------------------------
+// Hits of a bounce land anywhere in the scene. Shading them in the order of their position makes
+// the lanes of a wave gather nearby vertices and materials, and, because shading appends shadow
+// rays and continuations in its own order, the shadow rays and the next bounce's rays leave from
+// nearby points too, so the two traversals that follow walk shared subtrees. A count-bounded bucket
+// sort (the queue length is known only on the device): per-slice LDS histograms, a scan per bucket
+// over the slices, a scan over the buckets, and a scatter. Paths are keyed, so no result depends on
+// the order.
+__device__ __forceinline__ uint32_t spread3_4(uint32_t x) {  // 4 bits to every third position
+    x &= 0xfu;
+    x = (x | (x << 4)) & 0x0c3u;
+    x = (x | (x << 2)) & 0x249u;
+    return x;
+}
+__device__ __forceinline__ uint32_t grid_cell(float t, float cells) {
+    return (uint32_t)fminf(fmaxf(t * cells, 0.0f), cells - 1.0f);  // NaN -> 0
+}
+
+struct SortBox {
+    float lo[3], inv[3];
+};
+
+__device__ __forceinline__ uint32_t hit_bucket(const Queues& q, int cur, uint32_t i, const TreeView* trees,
+                                               const SortBox& box) {
+    const float4 h = q.hit[i];
+    if (__float_as_uint(h.y) == 0xFFFFFFFFu) return kSortBuckets - 1;  // misses: shade reads only the sky
+    const float4 r0 = q.ray[cur][2 * (size_t)i], r1 = q.ray[cur][2 * (size_t)i + 1];
+    const float p[3] = {r0.x + h.x * r1.x, r0.y + h.x * r1.y, r0.z + h.x * r1.z};
+    if (!trees) {  // 16^3 cells over the scene box, Morton order
+        uint32_t c[3];
+        for (int k = 0; k < 3; k++) c[k] = grid_cell((p[k] - box.lo[k]) * box.inv[k], 16.0f);
+        return spread3_4(c[0]) | (spread3_4(c[1]) << 1) | (spread3_4(c[2]) << 2);
+    }
+    // multi-scene waves: 8 scene classes (scene index mod 8) x 8^3 cells of that scene's box
+    const uint32_t sid = q.rsc[cur][i];
+    const float* b = trees[sid].bbox;
+    uint32_t m = 0;
+    for (int k = 0; k < 3; k++) {
+        const float w = b[3 + k] - b[k];
+        const uint32_t c = grid_cell(w > 0.0f ? (p[k] - b[k]) / w : 0.0f, 8.0f);
+        m |= spread3_4(c) << k;
+    }
+    return ((sid & 7u) << 9) | (m & 0x1ffu);
+}
+
+__global__ __launch_bounds__(256) void k_sort_count(int cur, const TreeView* trees, SortBox box, Queues q) {
+    __shared__ uint32_t h[kSortBuckets];
+    for (uint32_t j = threadIdx.x; j < kSortBuckets; j += 256) h[j] = 0;
+    __syncthreads();
+    const uint32_t n = q.count[cur];
+    const uint32_t chunk = (n + kSortBlocks - 1) / kSortBlocks;
+    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+        const uint32_t key = hit_bucket(q, cur, i, trees, box);
+        q.skey[i] = key;
+        atomicAdd(&h[key], 1u);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < kSortBuckets; j += 256) q.shist[blockIdx.x * kSortBuckets + j] = h[j];
+}
+
+// Per bucket: exclusive offsets of the slices within the bucket, and the bucket's total.
+__global__ __launch_bounds__(256) void k_sort_scan_slices(Queues q) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= kSortBuckets) return;
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < kSortBlocks; b++) {
+        const uint32_t v = q.shist[b * kSortBuckets + j];
+        q.shist[b * kSortBuckets + j] = run;
+        run += v;
+    }
+    q.sbase[j] = run;
+}
+
+// Exclusive scan of the bucket totals (one workgroup of 1024 lanes, 4 buckets each).
+__global__ __launch_bounds__(1024) void k_sort_scan_buckets(Queues q) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    uint32_t v[4], sum = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = q.sbase[4 * t + k];
+        sum += v[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t x = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (int k = 0; k < 4; k++) {
+        q.sbase[4 * t + k] = run;
+        run += v[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(int cur, Queues q) {
+    __shared__ uint32_t off[kSortBuckets];
+    for (uint32_t j = threadIdx.x; j < kSortBuckets; j += 256)
+        off[j] = q.sbase[j] + q.shist[blockIdx.x * kSortBuckets + j];
+    __syncthreads();
+    const uint32_t n = q.count[cur];
+    const uint32_t chunk = (n + kSortBlocks - 1) / kSortBlocks;
+    const uint32_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) q.sperm[atomicAdd(&off[q.skey[i]], 1u)] = i;
+}
+
 // ---- shade -------------------------------------------------------------------------------------
+// `order` (or null): the queue entries in the order to shade them.
 __global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__ scenes,
                                                     const ReqDev* __restrict__ reqs, uint32_t bounce, int cur,
-                                                    PathState ps, Queues q) {
+                                                    PathState ps, Queues q, const uint32_t* __restrict__ order) {
     const uint32_t n = q.count[cur];
     const int nxt = cur ^ 1;
     for (uint32_t base = blockIdx.x * kPtBlock; base < n; base += gridDim.x * kPtBlock) {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t k = base + threadIdx.x;
+        const uint32_t i = (order && k < n) ? order[k] : k;
         bool cont = false, sh0 = false, sh1 = false;
         V3 ro = v3(0, 0, 0), rd = v3(0, 0, 0), so = v3(0, 0, 0), s0d = v3(0, 0, 0), s1d = v3(0, 0, 0);
         float s0max = 0.f, s1max = 0.f;
@@ -501,8 +612,22 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
         if ((e = launch_trace(ta, false, false, s, L.num_cus)) != hipSuccess) return e;
         if (L.dump.rays) hipLaunchKernelGGL(k_dump, dim3(shade_blocks), dim3(kPtBlock), 0, s, b, cur, L.ps, L.q, L.dump);
         hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, L.q.count, nxt);
+        const uint32_t* order = nullptr;
+        if (L.sort_min && npaths >= L.sort_min && L.q.sperm) {
+            SortBox box;
+            for (int k = 0; k < 3; k++) {
+                const float w = L.trace.tree.bbox[3 + k] - L.trace.tree.bbox[k];
+                box.lo[k] = L.trace.tree.bbox[k];
+                box.inv[k] = w > 0.0f ? 1.0f / w : 0.0f;
+            }
+            hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(256), 0, s, cur, L.trees, box, L.q);
+            hipLaunchKernelGGL(k_sort_scan_slices, dim3(kSortBuckets / 256), dim3(256), 0, s, L.q);
+            hipLaunchKernelGGL(k_sort_scan_buckets, dim3(1), dim3(1024), 0, s, L.q);
+            hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(256), 0, s, cur, L.q);
+            order = L.q.sperm;
+        }
         hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.wave->reqs, b, cur, L.ps,
-                           L.q);
+                           L.q, order);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         TraceArgs sa = ta;
         sa.rays = L.q.sray;

@@ -78,6 +78,7 @@ PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_dep
     L.max_depth = max_depth;
     L.num_cus = acc->ctx->num_cus;
     L.transp = sc->host.transp;
+    L.sort_min = shade_sort_min(acc->ctx->tuning);
     return L;
 }
 

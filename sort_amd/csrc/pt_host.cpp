@@ -52,7 +52,9 @@ float lum(const float* c) { return 0.212671f * c[0] + 0.715160f * c[1] + 0.07216
 
 }  // namespace
 
-size_t PathWorkspace::bytes_for(uint32_t paths) { return (size_t)std::max(paths, 1u) * kPathBytes + 16 * 4 + 32; }
+size_t PathWorkspace::bytes_for(uint32_t paths) {
+    return (size_t)std::max(paths, 1u) * kPathBytes + 16 * 4 + 32 + ((size_t)kSortBlocks + 1) * kSortBuckets * 4;
+}
 
 int PathWorkspace::ensure(sortgpu_ctx* ctx, uint32_t paths) {
     if (paths <= cap) return SORTGPU_OK;
@@ -67,7 +69,9 @@ int PathWorkspace::ensure(sortgpu_ctx* ctx, uint32_t paths) {
         (rc = ws(mem, ctx, &q.pid[1], n)) || (rc = ws(mem, ctx, &q.rsc[0], n)) || (rc = ws(mem, ctx, &q.rsc[1], n)) ||
         (rc = ws(mem, ctx, &q.hit, n)) || (rc = ws(mem, ctx, &q.sray, 4 * n)) || (rc = ws(mem, ctx, &q.ssid, 2 * n)) ||
         (rc = ws(mem, ctx, &q.ssc, 2 * n)) || (rc = ws(mem, ctx, &q.socc, 2 * n)) ||
-        (rc = ws(mem, ctx, &q.count, 16)) || (rc = ws(mem, ctx, &q.totals, 4))) {
+        (rc = ws(mem, ctx, &q.count, 16)) || (rc = ws(mem, ctx, &q.totals, 4)) || (rc = ws(mem, ctx, &q.skey, n)) ||
+        (rc = ws(mem, ctx, &q.sperm, n)) || (rc = ws(mem, ctx, &q.shist, (size_t)kSortBlocks * kSortBuckets)) ||
+        (rc = ws(mem, ctx, &q.sbase, kSortBuckets))) {
         release();
         return rc;
     }

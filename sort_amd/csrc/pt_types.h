@@ -88,6 +88,17 @@ struct Queues {
     uint8_t* socc;      // shadow occlusion
     uint32_t* count;    // [0],[1] extension queue counts, [2] shadow count
     unsigned long long* totals;  // [0] extension rays, [1] shadow rays
+    // spatial order of a bounce's hits (bucket sort): bucket per entry, the order, per-block bucket
+    // counts (kSortBlocks x kSortBuckets) and bucket bases (kSortBuckets)
+    uint32_t* skey;
+    uint32_t* sperm;
+    uint32_t* shist;
+    uint32_t* sbase;
 };
+
+// Bucket sort of a bounce's hits: 4096 buckets (a 16^3 grid over the scene's box; in multi-scene
+// waves 8 scene classes x 8^3 cells), counted by kSortBlocks workgroups over contiguous slices.
+constexpr uint32_t kSortBuckets = 4096;
+constexpr uint32_t kSortBlocks = 512;
 
 }  // namespace sortgpu

@@ -395,6 +395,7 @@ class HipDevice : public Device {
         L.max_depth = depth;
         L.num_cus = ctx_->num_cus;
         L.transps = att ? d_transps_ : nullptr;
+        L.sort_min = shade_sort_min(ctx_->tuning);
         return L;
     }
 

@@ -29,7 +29,7 @@ static void layouts(void) {
     S(sortgpu_tuning);
     F(sortgpu_tuning, refill); F(sortgpu_tuning, lds_entries); F(sortgpu_tuning, blocks_per_cu);
     F(sortgpu_tuning, leaf_batch); F(sortgpu_tuning, leaf_min); F(sortgpu_tuning, reorder);
-    F(sortgpu_tuning, reserved);
+    F(sortgpu_tuning, shade_order); F(sortgpu_tuning, reserved);
     S(sortgpu_material);
     F(sortgpu_material, type); F(sortgpu_material, base); F(sortgpu_material, trans);
     F(sortgpu_material, roughness); F(sortgpu_material, ior); F(sortgpu_material, eta); F(sortgpu_material, k);

@@ -200,3 +200,24 @@ def test_per_sample_clamp_matches_oracle(gpu_ctx):
                       quad_radiance=loaded["quad_radiance"], env_radiance=loaded["env"], max_recursive_depth=5,
                       clamp=loaded["clamp"]).render(I.camera(**loaded["camera"]), 32, 32, 4, seed=5).cpu().numpy()
     assert np.all(g <= np.float32(0.5))
+
+
+@pytest.mark.parametrize("mat", ["lambert", "dielectric"])
+def test_shade_order_changes_no_result(gpu_ctx, mat):
+    """Shading each bounce's hits in spatial order (the bucket sort before k_shade) changes only
+    speed: the image is bitwise the one shaded in queue order."""
+    from sort_amd.batch import set_tuning
+    sc = scenes.spheres_scene(n_spheres=40, level=3) if mat == "lambert" else \
+        scenes.cornell_sphere(sphere_material=SPHERE_MATS["dielectric"])
+    acc, pt = _gpu(gpu_ctx, sc, 8)
+    cam = I.camera(**sc["camera"])
+    imgs = []
+    try:
+        for order in (1, 2, 0):  # never, always, default (a 128x128x32 wave is 2^19 paths: sorted)
+            set_tuning(gpu_ctx, shade_order=order)
+            print(f"[pt] shade_order={order}", flush=True)
+            imgs.append(pt.render(cam, 128, 128, 32, seed=5).cpu().numpy())
+    finally:
+        set_tuning(gpu_ctx)
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[2].view(np.uint32))
