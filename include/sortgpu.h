@@ -474,7 +474,10 @@ typedef struct sortgpu_scene_file_info {
     uint32_t nverts, ntris, nquads, nmaterials;
     const float* pos;              /* nverts x 3 */
     const float* normals;          /* nverts x 3 (TransformNormal of the normalised normal) */
-    const float* uv;               /* nverts x 2 */
+    const float* uv;               /* nverts x 2: as stored, or Mesh::GenUV's spherical mapping about
+                                      the mean vertex for a mesh stored without (core/mesh.cpp:60-74) */
+    const float* tangents;         /* nverts x 3: Mesh::GenSmoothTagent (core/mesh.cpp:42-108), the
+                                      normalised sum of the tangents of the triangles at each vertex */
     const uint32_t* idx;           /* ntris x 3 */
     const uint32_t* tri_material;  /* ntris: index into materials (a mesh's -1 or unknown id: the last entry) */
     const sortgpu_quad* quads;     /* nquads */

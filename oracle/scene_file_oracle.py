@@ -483,6 +483,73 @@ def _map(mat, sources):
     return out, ("approx" if approx else "mapped")
 
 
+def _normalize_rows(v):
+    """Vector::Normalize (math/vector3.h:104-109) of each row: divided by its float32 length
+    unless that is zero."""
+    v = v.astype(np.float32)
+    ln = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+    nz = ln != F(0.0)
+    v[nz] = v[nz] / ln[nz, None]
+    return v
+
+
+def gen_uv(p):
+    """Mesh::GenUV (core/mesh.cpp:60-74) for a mesh stored without texture coordinates: for each
+    world-space vertex, the normalised direction d from the mean vertex (a float32 running sum
+    divided by the count) mapped to (acos(clamp(d.y)) * INV_PI, phi * INV_TWOPI), phi = atan2(d.z,
+    d.x) lifted into [0, 2pi) (sphericalTheta / sphericalPhi, bxdf_utils.h:80-87; constants of
+    math/utils.h:27-33). acos and atan2 are taken in double and rounded to float32."""
+    n = len(p)
+    if n == 0:
+        return np.zeros((0, 2), np.float32)
+    c = np.cumsum(p.astype(np.float32), axis=0, dtype=np.float32)[-1]  # sequential float32 sum
+    c = (c / F(n)).astype(np.float32)
+    d = _normalize_rows(p - c)
+    y = np.clip(d[:, 1], F(-1.0), F(1.0))
+    theta = np.arccos(y.astype(np.float64)).astype(np.float32)
+    phi = np.arctan2(d[:, 2].astype(np.float64), d[:, 0].astype(np.float64)).astype(np.float32)
+    phi = np.where(phi < F(0.0), phi + F(2.0) * PI, phi).astype(np.float32)
+    return np.stack([theta * F(0.3183099), phi * F(0.15915494)], 1).astype(np.float32)
+
+
+def tri_tangents(p, uv, tris):
+    """Mesh::genTagentForTri (core/mesh.cpp:76-108) for every triangle (rows of vertex indices):
+    (dv2 dp1 - dv1 dp2) / det with du1 = u0 - u2, du2 = u1 - u2, dv1 = v0 - v2, dv2 = v1 - v2,
+    dp1 = p0 - p2, dp2 = p1 - p2, det = du1 dv2 - dv1 du2; for det == 0 the first axis of
+    coordinateSystem (math/vector3.h:171-180) about normalize(cross(p0 - p1, p0 - p2))."""
+    p0, p1, p2 = p[tris[:, 0]], p[tris[:, 1]], p[tris[:, 2]]
+    t0, t1, t2 = uv[tris[:, 0]], uv[tris[:, 1]], uv[tris[:, 2]]
+    du1, du2 = t0[:, 0] - t2[:, 0], t1[:, 0] - t2[:, 0]
+    dv1, dv2 = t0[:, 1] - t2[:, 1], t1[:, 1] - t2[:, 1]
+    det = du1 * dv2 - dv1 * du2
+    dp1, dp2 = p0 - p2, p1 - p2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        out = (dv2[:, None] * dp1 - dv1[:, None] * dp2) / det[:, None]
+        a, b = p0 - p1, p0 - p2
+        n = np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                      a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1).astype(np.float32)
+        n = _normalize_rows(n)
+        xz = F(1.0) / np.sqrt(n[:, 0] * n[:, 0] + n[:, 2] * n[:, 2])
+        yz = F(1.0) / np.sqrt(n[:, 1] * n[:, 1] + n[:, 2] * n[:, 2])
+        zero = np.zeros(len(n), np.float32)
+        ax = np.stack([-n[:, 2] * xz, zero, n[:, 0] * xz], 1)
+        ay = np.stack([zero, n[:, 2] * yz, -n[:, 1] * yz], 1)
+        frame = np.where((np.abs(n[:, 0]) > np.abs(n[:, 1]))[:, None], ax, ay)
+    return np.where((det == F(0.0))[:, None], frame, out).astype(np.float32)
+
+
+def smooth_tangents(p, uv, tris):
+    """Mesh::GenSmoothTagent (core/mesh.cpp:42-58): per vertex, the float32 sum in triangle order
+    of the tangents of the triangles that name it (once per corner), normalised."""
+    acc = np.zeros((len(p), 3), np.float32)
+    if len(tris):
+        t = tri_tangents(p, uv, tris.astype(np.int64))
+        flat = tris.astype(np.int64).reshape(-1)
+        # unbuffered: a vertex named again is added again, in index order (triangle order)
+        np.add.at(acc, flat, np.repeat(t, 3, axis=0))
+    return _normalize_rows(acc)
+
+
 def read_scene(data: bytes) -> dict:
     """The scene dictionary the C ABI loader returns for the same bytes (see module docstring)."""
     r = _R(data)
@@ -559,7 +626,7 @@ def read_scene(data: bytes) -> dict:
 
     if r.u32() != sid("verification bits"):
         raise ValueError("not a scene stream")
-    pos, nrm, uvs, idx, tmat = [], [], [], [], []
+    pos, nrm, uvs, tans, idx, tmat = [], [], [], [], [], []
     quads, qrad = [], []
     env = np.zeros(3, np.float32)
     camera = None
@@ -577,7 +644,7 @@ def read_scene(data: bytes) -> dict:
             for _ in range(r.u32()):
                 vc = r.u32()
                 if vc == sid("MeshVisual"):
-                    r.b8()
+                    has_uv = r.b8()
                     nv = r.u32()
                     v = np.frombuffer(bytes(r.take(32 * nv)), "<f4").reshape(nv, 8).astype(np.float32)
                     p, n, t = v[:, 0:3], v[:, 3:6].copy(), v[:, 6:8]
@@ -599,11 +666,13 @@ def read_scene(data: bytes) -> dict:
                     ny = n[:, 0] * inv[1] + n[:, 1] * inv[5] + n[:, 2] * inv[9]
                     nzz = n[:, 0] * inv[2] + n[:, 1] * inv[6] + n[:, 2] * inv[10]
                     nrm.append(np.stack([nx, ny, nzz], 1).astype(np.float32))
-                    uvs.append(t.copy())
                     nt = r.u32()
                     tr = np.frombuffer(bytes(r.take(16 * nt)), "<i4").reshape(nt, 4)
                     if nt and (tr[:, :3].astype(np.int64).min() < 0 or tr[:, :3].astype(np.int64).max() >= nv):
                         raise ValueError("vertex index out of range")
+                    t = t.copy() if has_uv else gen_uv(pos[-1])
+                    uvs.append(t)
+                    tans.append(smooth_tangents(pos[-1], t, tr[:, :3]))
                     idx.append((tr[:, :3].astype(np.uint32) + np.uint32(nv_total)))
                     mi = tr[:, 3]
                     tmat.append(np.where((mi >= 0) & (mi < nm), mi, nm).astype(np.uint32))
@@ -705,6 +774,7 @@ def read_scene(data: bytes) -> dict:
     idx_a = cat(idx, (0, 3), np.uint32)
     return {"pos": cat(pos, (0, 3), np.float32), "idx": idx_a, "quads": quads,
             "normals": cat(nrm, (0, 3), np.float32), "uv": cat(uvs, (0, 2), np.float32),
+            "tangents": cat(tans, (0, 3), np.float32),
             "tri_material": cat(tmat, (0,), np.uint32), "materials": materials, "material_names": names,
             "quad_radiance": np.array(qrad, np.float32).reshape(-1, 3), "env": tuple(float(x) for x in env),
             "ntris": int(len(idx_a)), "camera": cam, "config": cfg, "skipped": skipped}

@@ -84,7 +84,8 @@ class SceneFileInfo(ctypes.Structure):
                 ("fov", ctypes.c_float), ("camera_fov_y", ctypes.c_float), ("has_camera", ctypes.c_uint32),
                 ("nverts", ctypes.c_uint32), ("ntris", ctypes.c_uint32), ("nquads", ctypes.c_uint32),
                 ("nmaterials", ctypes.c_uint32), ("pos", ctypes.c_void_p), ("normals", ctypes.c_void_p),
-                ("uv", ctypes.c_void_p), ("idx", ctypes.c_void_p), ("tri_material", ctypes.c_void_p),
+                ("uv", ctypes.c_void_p), ("tangents", ctypes.c_void_p), ("idx", ctypes.c_void_p),
+                ("tri_material", ctypes.c_void_p),
                 ("quads", ctypes.c_void_p), ("quad_radiance", ctypes.c_void_p), ("materials", ctypes.c_void_p),
                 ("env_radiance", ctypes.c_float * 3), ("accelerator", ctypes.c_uint32),
                 ("accel_max_depth", ctypes.c_int32), ("accel_max_leaf", ctypes.c_int32),

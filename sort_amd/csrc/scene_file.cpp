@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -355,7 +356,7 @@ MapResult map_material(const FileMaterial& fm, const std::vector<std::pair<std::
 
 struct sortgpu_scene_file {
     sortgpu_scene_file_info info{};
-    std::vector<float> pos, normals, uv, quad_radiance;
+    std::vector<float> pos, normals, uv, tangents, quad_radiance;
     std::vector<uint32_t> idx, tri_material;
     std::vector<sortgpu_quad> quads;
     std::vector<sortgpu_material> materials;
@@ -465,9 +466,95 @@ void read_materials(Reader& r, sortgpu_scene_file& f, std::vector<FileMaterial>&
     f.names.push_back("default");
 }
 
-// Mesh::Serialize (core/mesh.cpp:111-180) then Mesh::ApplyTransform (mesh.cpp:26-40).
+// Vector::Normalize (math/vector3.h:104-109): divided by its length unless that is zero.
+void normalize_in_place(float* v) {
+    const float len = length3(v[0], v[1], v[2]);
+    if (len != 0.0f) {
+        v[0] /= len;
+        v[1] /= len;
+        v[2] /= len;
+    }
+}
+
+// Mesh::GenUV (core/mesh.cpp:60-74) for a mesh stored without texture coordinates: the direction
+// from the mean vertex, as (sphericalTheta / pi, sphericalPhi / 2pi) (bxdf_utils.h:80-87, the
+// constants of math/utils.h:27-33).
+void gen_uv(sortgpu_scene_file& f, uint32_t base, uint32_t nv) {
+    if (nv == 0) return;
+    const float* p = f.pos.data() + 3 * (size_t)base;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t i = 0; i < nv; i++)
+        for (int k = 0; k < 3; k++) c[k] = c[k] + p[3 * (size_t)i + k];
+    for (int k = 0; k < 3; k++) c[k] /= (float)nv;
+    float* uv = f.uv.data() + 2 * (size_t)base;
+    for (uint32_t i = 0; i < nv; i++) {
+        float d[3] = {p[3 * (size_t)i] - c[0], p[3 * (size_t)i + 1] - c[1], p[3 * (size_t)i + 2] - c[2]};
+        normalize_in_place(d);
+        const float theta = acosf(std::min(std::max(d[1], -1.0f), 1.0f));
+        float phi = atan2f(d[2], d[0]);
+        if (phi < 0.0f) phi = phi + 2.0f * 3.1415926f;
+        uv[2 * (size_t)i] = theta * 0.3183099f;
+        uv[2 * (size_t)i + 1] = phi * 0.15915494f;
+    }
+}
+
+// Mesh::genTagentForTri (core/mesh.cpp:76-108): dp/du of the triangle from its texture
+// coordinates, or, when they are degenerate, coordinateSystem's first axis (math/vector3.h:171-180)
+// about the face normal.
+void tri_tangent(const float* p0, const float* p1, const float* p2, const float* t0, const float* t1,
+                 const float* t2, float* out) {
+    const float du1 = t0[0] - t2[0], du2 = t1[0] - t2[0];
+    const float dv1 = t0[1] - t2[1], dv2 = t1[1] - t2[1];
+    const float det = du1 * dv2 - dv1 * du2;
+    if (det == 0.0f) {
+        const float a[3] = {p0[0] - p1[0], p0[1] - p1[1], p0[2] - p1[2]};
+        const float b[3] = {p0[0] - p2[0], p0[1] - p2[1], p0[2] - p2[2]};
+        float n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        normalize_in_place(n);  // normalize() (vector3.h:163-168) gives the same floats
+        if (std::fabs(n[0]) > std::fabs(n[1])) {
+            const float inv = 1.0f / std::sqrt(n[0] * n[0] + n[2] * n[2]);
+            out[0] = -n[2] * inv;
+            out[1] = 0.0f;
+            out[2] = n[0] * inv;
+        } else {
+            const float inv = 1.0f / std::sqrt(n[1] * n[1] + n[2] * n[2]);
+            out[0] = 0.0f;
+            out[1] = n[2] * inv;
+            out[2] = -n[1] * inv;
+        }
+        return;
+    }
+    for (int k = 0; k < 3; k++) {
+        const float dp1 = p0[k] - p2[k], dp2 = p1[k] - p2[k];
+        out[k] = (dv2 * dp1 - dv1 * dp2) / det;
+    }
+}
+
+// Mesh::GenSmoothTagent (core/mesh.cpp:42-58): each vertex's tangent is the normalised sum, in
+// triangle order, of the tangents of the triangles that use it (world-space positions; the
+// transform is not applied to tangents).
+void gen_smooth_tangents(sortgpu_scene_file& f, uint32_t base, uint32_t nv, uint32_t first_tri) {
+    const size_t t0 = 3 * (size_t)base;
+    f.tangents.resize(t0 + 3 * (size_t)nv, 0.0f);
+    float* tan = f.tangents.data();
+    const float* pos = f.pos.data();
+    const float* uv = f.uv.data();
+    const size_t ntri = f.idx.size() / 3;
+    for (size_t t = first_tri; t < ntri; t++) {
+        const uint32_t* v = f.idx.data() + 3 * t;
+        float tt[3];
+        tri_tangent(pos + 3 * (size_t)v[0], pos + 3 * (size_t)v[1], pos + 3 * (size_t)v[2], uv + 2 * (size_t)v[0],
+                    uv + 2 * (size_t)v[1], uv + 2 * (size_t)v[2], tt);
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < 3; k++) tan[3 * (size_t)v[c] + k] += tt[k];
+    }
+    for (uint32_t i = 0; i < nv; i++) normalize_in_place(tan + t0 + 3 * (size_t)i);
+}
+
+// Mesh::Serialize (core/mesh.cpp:111-180), then MeshVisual::ApplyTransform (entity/visual.cpp:35-39):
+// Mesh::ApplyTransform (mesh.cpp:26-40), Mesh::GenUV, Mesh::GenSmoothTagent.
 void read_mesh(Reader& r, sortgpu_scene_file& f, const Transform& tr, uint32_t nmats) {
-    (void)r.b8("mesh has uv");
+    const bool has_uv = r.b8("mesh has uv");
     const uint32_t nv = r.u32("vertex count");
     if ((uint64_t)nv * 32u > r.remaining()) r.fail("vertex count", "truncated");
     f.pos.reserve(f.pos.size() + 3 * (size_t)nv);
@@ -532,6 +619,8 @@ void read_mesh(Reader& r, sortgpu_scene_file& f, const Transform& tr, uint32_t n
         r.fail("volume tag", "unknown volume tag");
     }
     if (r.u32("end of mesh") != sid("end of mesh")) r.fail("end of mesh", "missing end-of-mesh tag");
+    if (!has_uv) gen_uv(f, base, nv);
+    gen_smooth_tangents(f, base, nv, (uint32_t)(f.idx.size() / 3) - nt);
 }
 
 // HairVisual::Serialize (entity/visual.cpp:44-100): read and counted, not carried.
@@ -687,6 +776,7 @@ void finish(sortgpu_scene_file& f) {
     i.pos = f.pos.data();
     i.normals = f.normals.data();
     i.uv = f.uv.data();
+    i.tangents = f.tangents.data();
     i.idx = f.idx.data();
     i.tri_material = f.tri_material.data();
     i.quads = f.quads.data();

@@ -64,6 +64,7 @@ class SceneFile:
         pos = arr(i.pos, i.nverts * 3, np.float32, (i.nverts, 3))
         normals = arr(i.normals, i.nverts * 3, np.float32, (i.nverts, 3))
         uv = arr(i.uv, i.nverts * 2, np.float32, (i.nverts, 2))
+        tangents = arr(i.tangents, i.nverts * 3, np.float32, (i.nverts, 3))
         idx = arr(i.idx, i.ntris * 3, np.uint32, (i.ntris, 3))
         tri_material = arr(i.tri_material, i.ntris, np.uint32, (i.ntris,))
         qa = arr(i.quads, i.nquads * 34, np.float32, (i.nquads, 34))
@@ -82,7 +83,8 @@ class SceneFile:
             camera = dict(eye=tuple(i.eye), target=tuple(i.target), up=tuple(i.up),
                           fov_y_deg=math.degrees(i.camera_fov_y))
         return {
-            "pos": pos, "idx": idx, "quads": quads, "normals": normals, "uv": uv, "tri_material": tri_material,
+            "pos": pos, "idx": idx, "quads": quads, "normals": normals, "uv": uv, "tangents": tangents,
+            "tri_material": tri_material,
             "materials": materials, "material_names": names, "quad_radiance": quad_radiance,
             "env": tuple(i.env_radiance), "ntris": int(i.ntris), "camera": camera, "clamp": float(i.clamping),
             "config": {"threads": i.threads, "spp": i.spp, "width": i.image_width, "height": i.image_height,

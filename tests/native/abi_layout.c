@@ -70,7 +70,8 @@ static void layouts(void) {
     F(sortgpu_scene_file_info, fov); F(sortgpu_scene_file_info, camera_fov_y); F(sortgpu_scene_file_info, has_camera);
     F(sortgpu_scene_file_info, nverts); F(sortgpu_scene_file_info, ntris); F(sortgpu_scene_file_info, nquads);
     F(sortgpu_scene_file_info, nmaterials); F(sortgpu_scene_file_info, pos); F(sortgpu_scene_file_info, normals);
-    F(sortgpu_scene_file_info, uv); F(sortgpu_scene_file_info, idx); F(sortgpu_scene_file_info, tri_material);
+    F(sortgpu_scene_file_info, uv); F(sortgpu_scene_file_info, tangents);
+    F(sortgpu_scene_file_info, idx); F(sortgpu_scene_file_info, tri_material);
     F(sortgpu_scene_file_info, quads); F(sortgpu_scene_file_info, quad_radiance);
     F(sortgpu_scene_file_info, materials); F(sortgpu_scene_file_info, env_radiance);
     F(sortgpu_scene_file_info, accelerator); F(sortgpu_scene_file_info, accel_max_depth);

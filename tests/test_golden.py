@@ -25,9 +25,19 @@ def test_scene_small_fixture(reader):
     data = open(os.path.join(HERE, "scene_small.sort"), "rb").read()
     want = json.load(open(os.path.join(HERE, "scene_small.json")))
     sc = scene_file.parse(data) if reader == "library" else SF.read_scene(data)
-    for k in ("pos", "idx", "normals", "uv", "tri_material", "quad_radiance"):
+    for k in ("pos", "idx", "normals", "tri_material", "quad_radiance"):
         assert list(np.asarray(sc[k]).shape) == want[k]["shape"], k
         assert _digest(sc[k]) == want[k]["sha256"], k
+    # one mesh stores texture coordinates, the other has them generated (acos / atan2 of the
+    # platform's libm): within two ulps; the tangents smoothed from them within 1e-5
+    uv = np.float32(want["uv_values"])
+    assert sc["uv"].shape == uv.shape
+    assert np.all(np.abs(sc["uv"] - uv) <= 2 * np.spacing(np.maximum(np.abs(sc["uv"]), np.abs(uv))))
+    tan = np.array([[np.nan if x is None else x for x in row] for row in want["tangent_values"]], np.float32)
+    assert sc["tangents"].shape == tan.shape
+    assert np.array_equal(np.isnan(sc["tangents"]), np.isnan(tan))
+    ok = ~np.isnan(tan)
+    assert np.abs(sc["tangents"][ok] - tan[ok]).max() < 1e-5
     assert len(sc["quads"]) == len(want["quads"])
     for q, w in zip(sc["quads"], want["quads"]):
         assert _digest(np.float32(q["mat"])) == w["mat"] and _digest(np.float32(q["inv"])) == w["inv"]

@@ -1,5 +1,6 @@
-"""A scene read from a .sort file renders on the GPU like the oracle renders the same loaded scene,
-and like the built-in scene it was written from (same geometry and lights; normals renormalised).
+"""A scene read from a .sort file renders on the GPU like the oracle renders the same loaded scene
+(its smoothed tangents included), and, without the tangents, like the built-in scene it was
+written from (same geometry and lights; normals renormalised).
 """
 import numpy as np
 import pytest
@@ -18,7 +19,8 @@ def _lum(img):
 def _render(ctx, sc, W, H, spp, depth, seed):
     acc = Obvh(ctx).build(sc["pos"], sc["idx"], sc["quads"])
     pt = I.PathTracing(acc, sc["materials"], tri_material=sc["tri_material"], normals=sc["normals"],
-                       quad_radiance=sc["quad_radiance"], env_radiance=sc["env"], max_recursive_depth=depth)
+                       tangents=sc.get("tangents"), quad_radiance=sc["quad_radiance"], env_radiance=sc["env"],
+                       max_recursive_depth=depth)
     img = pt.render(I.camera(**sc["camera"]), W, H, spp, seed=seed).cpu().numpy().reshape(-1, 3)
     pt.close()
     return img
@@ -41,7 +43,9 @@ def test_loaded_scene_renders_like_oracle_and_source(gpu_ctx, tmp_path, sphere):
     assert close.mean() > 0.97
     a, b = _lum(g), _lum(o)
     assert float(np.mean((a - b) ** 2 / (b * b + 1e-3))) < 1e-4
-    # the scene it was written from: same keys, so the same paths up to the renormalised normals
+    # the scene it was written from (edge tangents): same keys, so the same paths up to the
+    # renormalised normals when the loaded scene drops its tangents too
     s = _render(gpu_ctx, src, W, H, spp, depth, seed=11)
-    a, b = _lum(g), _lum(s)
+    e = _render(gpu_ctx, {k: v for k, v in sc.items() if k != "tangents"}, W, H, spp, depth, seed=11)
+    a, b = _lum(e), _lum(s)
     assert float(np.mean((a - b) ** 2 / (b * b + 1e-3))) < 1e-3

@@ -18,7 +18,7 @@ import pytest
 from oracle import scene_file_oracle as O
 from sort_amd import SortGpuError, scene_file, scenes
 
-ARRAYS = ("pos", "idx", "normals", "uv", "tri_material", "quad_radiance")
+ARRAYS = ("pos", "idx", "normals", "tri_material", "quad_radiance")
 
 
 def _same(a, b):
@@ -26,9 +26,28 @@ def _same(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
+def _within_ulp(a, b, n=1):
+    a, b = np.float32(a), np.float32(b)
+    return a.shape == b.shape and bool(np.all(np.abs(a - b) <= n * np.spacing(np.maximum(np.abs(a), np.abs(b)))))
+
+
+def _assert_generated(g, o):
+    """Texture coordinates and tangents. Stored coordinates are copied and generated ones come
+    from acosf / atan2f of the host's libm in the loader and from double rounded to float in the
+    oracle: an ulp of the angle, scaled by 1/pi or 1/2pi, is at most two ulps of the coordinate.
+    The tangents are the oracle's smoothing of the loader's own coordinates, bit for bit, and
+    equal the oracle's wherever the coordinates are."""
+    assert _within_ulp(g["uv"], o["uv"], 2)
+    assert _same(g["tangents"], O.smooth_tangents(g["pos"], g["uv"], g["idx"]))
+    if _same(g["uv"], o["uv"]):
+        assert _same(g["tangents"], o["tangents"])
+    assert np.array_equal(np.isnan(g["tangents"]), np.isnan(o["tangents"]))
+
+
 def _assert_equal_scenes(g, o):
     for k in ARRAYS:
         assert _same(g[k], o[k]), k
+    _assert_generated(g, o)
     assert len(g["quads"]) == len(o["quads"])
     for qg, qo in zip(g["quads"], o["quads"]):
         assert _same(np.float32(qg["mat"]), np.float32(qo["mat"])) and _same(np.float32(qg["inv"]), np.float32(qo["inv"]))
@@ -90,7 +109,7 @@ def test_closed_form_transforms_and_lights():
     m = np.array([[2, 0, 0, 1], [0, 4, 0, 2], [0, 0, 0.5, 3], [0, 0, 0, 1]], np.float32)
     light_m = np.array([[0, 0, 3, 5], [2, 0, 0, 6], [0, 1, 0, 7], [0, 0, 0, 1]], np.float32)  # columns scaled 2, 1, 3
     data = O.write_scene(materials=[{"name": "white", "kind": O.DIFFUSE, "params": {"Diffuse": (0.5, 0.25, 1.0)}}],
-                         meshes=[{"verts": verts, "tris": tris, "matrix": m}],
+                         meshes=[{"verts": verts, "tris": tris, "matrix": m, "has_uv": True}],
                          lights=[{"type": "AREA", "matrix": light_m, "energy": 10.0, "color": (1.0, 0.5, 0.25),
                                   "shape": "RECTANGLE", "size": 0.5, "size_y": 0.25}])
     for sc in (O.read_scene(data), scene_file.parse(data)):
@@ -99,6 +118,8 @@ def test_closed_form_transforms_and_lights():
         # normal (0,0,2) -> normalised (0,0,1) -> transposed inverse: (0, 0, 1/0.5) = (0, 0, 2)
         assert np.array_equal(sc["normals"], np.tile(np.float32([0, 0, 2]), (4, 1)))
         assert np.array_equal(sc["uv"], verts[:, 6:8])
+        # uv = (x, y) before the transform: dp/du is the transform's first column, unit length
+        assert np.array_equal(sc["tangents"], np.tile(np.float32([1, 0, 0]), (4, 1)))
         assert sc["idx"].tolist() == [[0, 1, 2], [0, 2, 3]] and sc["tri_material"].tolist() == [0, 0]
         q = sc["quads"][0]
         # the light's scale (column lengths 2, 1, 3) moves into the shape: unit columns remain
@@ -126,6 +147,72 @@ def test_camera_field_of_view_closed_forms():
             assert math.radians(sc["camera"]["fov_y_deg"]) == pytest.approx(want_y, rel=1e-6), (fit, w, h)
             assert sc["env"] == (0.5, 0.5, 0.5)  # no light at all: SORT's default sky
             assert sc["skipped"] == {}
+
+
+def test_generated_uv_closed_forms():
+    """Mesh::GenUV on an octahedron about the origin: the six axis directions have known
+    spherical angles (theta from +y, phi from +x towards +z)."""
+    axes = np.float32([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]])
+    verts = np.zeros((6, 8), np.float32)
+    verts[:, :3] = axes
+    verts[:, 6:8] = 7.0  # stored coordinates that has_uv = False discards
+    tris = np.int32([[0, 2, 4, 0], [4, 2, 1, 0], [1, 2, 5, 0], [5, 2, 0, 0]])
+    want = [(0.5, 0.0), (0.5, 0.5), (0.0, 0.0), (1.0, 0.0), (0.5, 0.25), (0.5, 0.75)]
+    data = O.write_scene(meshes=[{"verts": verts, "tris": tris, "has_uv": False}])
+    for sc in (O.read_scene(data), scene_file.parse(data)):
+        assert np.abs(sc["uv"] - np.float32(want)).max() < 1e-6
+    stored = scene_file.parse(O.write_scene(meshes=[{"verts": verts, "tris": tris, "has_uv": True}]))
+    assert np.all(stored["uv"] == 7.0)
+
+
+def _tangents(verts, tris, matrix=None):
+    mesh = {"verts": np.float32(verts), "tris": np.int32(tris), "has_uv": True}
+    if matrix is not None:
+        mesh["matrix"] = np.float32(matrix)
+    data = O.write_scene(meshes=[mesh])
+    o, g = O.read_scene(data), scene_file.parse(data)
+    _assert_generated(g, o)
+    return g["tangents"]
+
+
+def _v(p, uv):
+    return list(p) + [0.0, 0.0, 1.0] + list(uv)
+
+
+def test_smooth_tangent_closed_forms():
+    """Mesh::genTagentForTri / GenSmoothTagent against closed forms: dp/du of an affine image of
+    the uv plane, the sum at a shared vertex, coordinateSystem's axis for degenerate coordinates
+    and NaN for a degenerate triangle (a zero normal in coordinateSystem, as in the reference)."""
+    g = np.random.default_rng(5)
+    # a uv grid placed by an affine transform: every triangle's dp/du is the first column
+    xs = np.float32(np.linspace(0.0, 1.0, 4))
+    verts = [_v((x, y, 0.0), (x, y)) for y in xs for x in xs]
+    tris = [[r * 4 + c, r * 4 + c + 1, r * 4 + c + 5, 0] for r in range(3) for c in range(3)] + \
+           [[r * 4 + c, r * 4 + c + 5, r * 4 + c + 4, 0] for r in range(3) for c in range(3)]
+    for _ in range(5):
+        m = np.eye(4, dtype=np.float32)
+        m[:3, :3] = g.normal(size=(3, 3))
+        m[:3, 3] = g.normal(size=3)
+        col = m[:3, 0] / np.linalg.norm(m[:3, 0])
+        t = _tangents(verts, tris, m)
+        assert np.abs(t - col).max() < 1e-5
+    # a vertex shared by two triangles takes the normalised sum of their tangents
+    verts = [_v((0, 0, 0), (0, 0)), _v((1, 0, 0), (1, 0)), _v((0, 1, 0), (0, 1)), _v((0, 0, 2), (1, 0))]
+    t = _tangents(verts, [[0, 1, 2, 0], [0, 3, 2, 0]])  # dp/du = (1, 0, 0) and (0, 0, 2)
+    s5 = 1 / math.sqrt(5.0)
+    assert np.abs(t - np.float32([[s5, 0, 2 * s5], [1, 0, 0], [s5, 0, 2 * s5], [0, 0, 1]])).max() < 1e-6
+    # equal texture coordinates: coordinateSystem's first axis about the face normal
+    p = np.float32([[0, 0, 0], [1, 0, 0.5], [0, 1, 2]])
+    n = np.cross(p[0] - p[1], p[0] - p[2]).astype(np.float64)
+    n /= np.linalg.norm(n)
+    want = (np.array([-n[2], 0, n[0]]) / math.hypot(n[0], n[2]) if abs(n[0]) > abs(n[1])
+            else np.array([0, n[2], -n[1]]) / math.hypot(n[1], n[2]))
+    t = _tangents([_v(q, (0.3, 0.3)) for q in p], [[0, 1, 2, 0]])
+    assert np.abs(t - want).max() < 1e-6
+    # a triangle with two equal corners has no normal: its vertices' tangents are NaN
+    t = _tangents([_v((0, 0, 0), (0, 0)), _v((1, 0, 0), (1, 0)), _v((1, 0, 0), (1, 0)), _v((0, 1, 0), (0, 1))],
+                  [[0, 1, 2, 0], [0, 1, 3, 0]])
+    assert np.isnan(t[:3]).all() and np.abs(t[3] - np.float32([1, 0, 0])).max() < 1e-6
 
 
 def test_materials_map_to_gpu_records():
