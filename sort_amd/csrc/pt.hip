@@ -7,8 +7,9 @@
 //   shade    emission at the first vertex, NEE (up to two shadow rays per vertex, written to
 //            per-path slots), BSDF continuation, Russian roulette, compaction of survivors by
 //            wave ballot + one atomic per wave
-//   shadow   any-hit traversal of the compacted shadow queue
-//   resolve  zero the contribution slot of every occluded shadow ray
+//   shadow   any-hit traversal of the compacted shadow queue; an occluded ray clears its
+//            contribution slot from inside the traversal (with transparent materials: the
+//            attenuation walk, then resolve scales or clears each slot)
 //   finalize fold the pending contributions, average each pixel's valid samples in sample order
 //            (image_evaluation.cpp:222-244: NaN/inf samples dropped from the mean)
 //
@@ -489,17 +490,6 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__
 }
 
 // ---- resolve -----------------------------------------------------------------------------------
-__global__ __launch_bounds__(kPtBlock) void k_resolve(PathState ps, Queues q) {
-    const uint32_t n = q.count[2];
-    for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
-        if (q.socc[i]) {
-            const uint32_t slot = q.ssid[i];
-            float4* c = (slot & 1u) ? ps.c1 : ps.c0;
-            c[slot >> 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-    }
-}
-
 // Transparent shadows: every shadow ray's attenuation (r, g, b, surfaces) was written over the
 // first half of its ray record; a black one clears its contribution slot, any other scales it
 // (EvaluateDirect's attenuation factor, integratormethod.cpp:134-142 and 181-185; applied here to
@@ -647,10 +637,13 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
             if ((e = launch_attenuation(sa, s, L.num_cus)) != hipSuccess) return e;
             hipLaunchKernelGGL(k_resolve_att, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.ps, L.q);
         } else {
-            sa.out = L.q.socc;
+            // occluded shadow rays clear their contribution slots inside the any-hit launch
+            sa.out = nullptr;
+            sa.occ_slot = L.q.ssid;
+            sa.occ_c0 = L.ps.c0;
+            sa.occ_c1 = L.ps.c1;
             if (L.trees) sa.ray_scene = L.q.ssc;
             if ((e = launch_trace(sa, true, false, s, L.num_cus)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_resolve, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.ps, L.q);
         }
         hipLaunchKernelGGL(k_add_totals, dim3(1), dim3(64), 0, s, L.q.count, cur, L.q.totals);
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -68,7 +68,7 @@ int PathWorkspace::ensure(sortgpu_ctx* ctx, uint32_t paths) {
         (rc = ws(mem, ctx, &q.ray[1], 2 * n)) || (rc = ws(mem, ctx, &q.pid[0], n)) ||
         (rc = ws(mem, ctx, &q.pid[1], n)) || (rc = ws(mem, ctx, &q.rsc[0], n)) || (rc = ws(mem, ctx, &q.rsc[1], n)) ||
         (rc = ws(mem, ctx, &q.hit, n)) || (rc = ws(mem, ctx, &q.sray, 4 * n)) || (rc = ws(mem, ctx, &q.ssid, 2 * n)) ||
-        (rc = ws(mem, ctx, &q.ssc, 2 * n)) || (rc = ws(mem, ctx, &q.socc, 2 * n)) ||
+        (rc = ws(mem, ctx, &q.ssc, 2 * n)) ||
         (rc = ws(mem, ctx, &q.count, 16)) || (rc = ws(mem, ctx, &q.totals, 4)) || (rc = ws(mem, ctx, &q.skey, n)) ||
         (rc = ws(mem, ctx, &q.sperm, n)) || (rc = ws(mem, ctx, &q.shist, (size_t)kSortBlocks * kSortBuckets)) ||
         (rc = ws(mem, ctx, &q.sbase, kSortBuckets))) {

@@ -85,7 +85,6 @@ struct Queues {
     float4* sray;       // shadow rays
     uint32_t* ssid;     // shadow slot: 2 * path + k
     uint32_t* ssc;      // scene of each shadow entry
-    uint8_t* socc;      // shadow occlusion
     uint32_t* count;    // [0],[1] extension queue counts, [2] shadow count
     unsigned long long* totals;  // [0] extension rays, [1] shadow rays
     // spatial order of a bounce's hits (bucket sort): bucket per entry, the order, per-block bucket

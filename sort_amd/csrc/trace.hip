@@ -218,7 +218,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                         sp = 1;
                     } else {
                         if (TRANSP) reinterpret_cast<float4*>(a.out)[(size_t)my * ostride] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-                        else if (ANY) static_cast<uint8_t*>(a.out)[my] = 0;
+                        else if (ANY) {
+                            if (!a.occ_slot) static_cast<uint8_t*>(a.out)[my] = 0;
+                        }
                         else reinterpret_cast<float4*>(a.out)[my] =
                                  make_float4(FLT_MAX, __uint_as_float(0xffffffffu), 0.f, 0.f);
                         ray_id = 0xffffffffu;
@@ -365,8 +367,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             }
         }
         if (!TRANSP && ray_id != 0xffffffffu && sp == 0 && li == 0xffffffffu) {
-            if (ANY) static_cast<uint8_t*>(a.out)[ray_id] = occluded ? 1 : 0;
-            else reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);
+            if (ANY) {
+                if (!a.occ_slot) {
+                    static_cast<uint8_t*>(a.out)[ray_id] = occluded ? 1 : 0;
+                } else if (occluded) {  // the shadow ray's contribution slot is cleared here
+                    const uint32_t slot = a.occ_slot[ray_id];
+                    float4* c = static_cast<float4*>((slot & 1u) ? a.occ_c1 : a.occ_c0);
+                    c[slot >> 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            } else {
+                reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);
+            }
             ray_id = 0xffffffffu;
         }
         occluded = false;

@@ -39,6 +39,12 @@ struct TraceArgs {
     // transparent-shadow attenuation only: ray i's result goes to float4 index i * out_stride of out
     // (0 or 1: packed; 2: over the first half of ray i's own 32-byte record when out == rays)
     uint32_t out_stride = 0;
+    // any-hit only, for the path tracer's shadow rays: with occ_slot set, an occluded ray i clears
+    // the 16-byte contribution slot occ_slot[i] (an odd slot s in occ_c1, an even one in occ_c0,
+    // at index s / 2) and nothing is written to out; an unoccluded ray writes nothing
+    const uint32_t* occ_slot = nullptr;
+    void* occ_c0 = nullptr;
+    void* occ_c1 = nullptr;
 };
 
 // Launches the closest (any = false) or any-hit traversal on `stream`.
