@@ -26,7 +26,8 @@ Further legs, reported beside the headline under their own keys (--legs selects 
   batched  -- 64 scenes in one concatenated pool, one persistent launch over all their incoherent
               rays, against one launch per scene;
   served   -- the render server under Poisson arrivals (20 requests/s over 16 resident scenes) and
-              saturated, p50/p99 latency and throughput against one request at a time.
+              saturated, p50/p99 latency and throughput against one request at a time; both use
+              trees of the node width the autotuner picks on a sample of the served renders.
 """
 from __future__ import annotations
 
@@ -56,6 +57,8 @@ def parse():
     p.add_argument("--tris", type=int, default=1 << 20)
     p.add_argument("--legs", default="intersection,attenuation,multi,film,pt,batched,served")
     p.add_argument("--served-requests", type=int, default=60)
+    p.add_argument("--served-width", type=int, default=0, choices=[0, 4, 8],
+                   help="node width of the served scenes' trees (0: the autotuner's pick on a sample of the renders)")
     p.add_argument("--shade-order", type=int, default=0, choices=[0, 1, 2],
                    help="path tracer: shade each bounce's hits in spatial order (0 default, 1 never, 2 always)")
     p.add_argument("--copy-probe", action="store_true",
@@ -544,8 +547,11 @@ def main():
         out["batched"] = leg_batched(ctx)
         set_tuning(ctx, shade_order=a.shade_order)  # the batched leg leaves the defaults applied
     if "served" in legs:
-        from sort_amd.serve import bench_served
-        out["served"] = bench_served(ctx, n_requests=a.served_requests, log=log)
+        from sort_amd.serve import bench_served, choose_width
+        width, width_ms = (a.served_width, None) if a.served_width else choose_width(ctx, log=log)
+        set_tuning(ctx, shade_order=a.shade_order)  # the width choice leaves the defaults applied
+        out["served"] = bench_served(ctx, n_requests=a.served_requests, width=width, log=log)
+        out["served"]["server"]["width_sample_ms"] = width_ms
     if a.copy_probe:
         # a device-to-device copy of known size: its HBM counters per dispatch calibrate the
         # traversal's (FETCH_SIZE / WRITE_SIZE runs of a profiler over this program)

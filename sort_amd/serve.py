@@ -217,14 +217,14 @@ def run_served(server: Server, trace, scene_ids, time_scale=1.0, on_submit=None)
     return ids, out, wall
 
 
-def run_one_at_a_time(pts, trace, time_scale=1.0, per_job=None):
+def run_one_at_a_time(pts, trace, time_scale=1.0, per_job=None, width=8):
     """The plain path: each request rendered alone (``PathTracing.render``, result copied to the
     host) in arrival order, starting when it has arrived and the previous one is done. With
     `per_job` = (ctx, built scenes), each request first builds and uploads its scene and frees it
     afterwards, as one SORT render job does (the scene is loaded and its accelerator built per
-    job); otherwise `pts` holds every scene resident. Returns (latencies in seconds, wall seconds,
-    ray totals)."""
-    from .accel import Obvh
+    job); otherwise `pts` holds every scene resident. `width`: node width of the per-job trees.
+    Returns (latencies in seconds, wall seconds, ray totals)."""
+    from .accel import Obvh, Qbvh
     from .integrator import PathTracing
     t0 = time.perf_counter()
     lat, ext, shadow = [], 0, 0
@@ -236,7 +236,7 @@ def run_one_at_a_time(pts, trace, time_scale=1.0, per_job=None):
         if per_job:
             ctx, built = per_job
             sc = built[r["scene"]][1]
-            acc = Obvh(ctx).build(sc["pos"], sc["idx"], quads=sc["quads"])
+            acc = (Obvh if width == 8 else Qbvh)(ctx).build(sc["pos"], sc["idx"], quads=sc["quads"])
             pt = PathTracing(acc, sc["materials"], tri_material=sc["tri_material"], quad_radiance=sc["quad_radiance"],
                              env_radiance=sc["env"])
         else:
@@ -252,17 +252,56 @@ def run_one_at_a_time(pts, trace, time_scale=1.0, per_job=None):
     return np.array(lat), time.perf_counter() - t0, (ext, shadow)
 
 
+def choose_width(ctx, scale=1.0, n_sample=4, log=print):
+    """Node width for the served scenes, picked by the autotuner (autotune.tune over the two
+    widths, default knobs otherwise) on a sample of the served workload: one 512x512, 16-spp render
+    (4M paths, enough to fill the GPU) on each of the `n_sample` largest scenes, trees of each
+    width. Returns (width, {width: ms})."""
+    from . import autotune
+    from .integrator import PathTracing
+    sizes = scene_sizes()
+    pick = [int(i) for i in np.argsort(-sizes)[:n_sample]]
+    g = np.random.default_rng(12345)
+    cams = [random_camera(g) for _ in pick]
+    runners, keep = {}, []
+    for w in (8, 4):
+        built = build_scenes(ctx, scale=scale, width=w)
+        pts = [PathTracing(built[i][0], built[i][1]["materials"], tri_material=built[i][1]["tri_material"],
+                           quad_radiance=built[i][1]["quad_radiance"], env_radiance=built[i][1]["env"]) for i in pick]
+        keep.append((built, pts))
+
+        def run(pts=pts):
+            for pt, cam in zip(pts, cams):
+                pt.render(cam, 512, 512, 16, seed=7)
+        runners[w] = run
+    best, results = autotune.tune(ctx, runners, lds=(0,), refills=(0,), blocks=(0,), leaf_batches=(0,), leaf_mins=(0,),
+                                  reps=3)
+    for built, pts in keep:
+        for pt in pts:
+            pt.close()
+        for acc, _ in built:
+            acc.release()
+    ms = {int(r["width"]): round(r["ms"], 2) for r in results}
+    log(f"[served] node width {best['width']} (sample render ms by width: {ms})")
+    return int(best["width"]), ms
+
+
 def _pct(x, q):
     return float(np.percentile(np.asarray(x), q))
 
 
-def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, inflight_waves=0, graphs=False, log=print):
-    """Served workload, both ways. Returns a dict of p50/p99 latency and throughput for the
+def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, inflight_waves=0, graphs=False, width=0,
+                 log=print):
+    """Served workload, both ways, over trees of node width `width` (0: chosen by choose_width;
+    both ways use the same trees). Returns a dict of p50/p99 latency and throughput for the
     Poisson-arrival run and for a saturated run (every request submitted at once), served versus
     one request at a time. Images are not compared here (the tests do that)."""
     from .integrator import PathTracing
+    width_ms = None
+    if width == 0:
+        width, width_ms = choose_width(ctx, scale=scale, log=log)
     t = time.perf_counter()
-    built = build_scenes(ctx, scale=scale)
+    built = build_scenes(ctx, scale=scale, width=width)
     log(f"[served] built {len(built)} scenes ({sum(sc['ntris'] for _, sc in built)} tris) in "
         f"{time.perf_counter() - t:.1f}s")
     trace = request_trace(n_requests, rate=rate)
@@ -293,7 +332,7 @@ def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, infligh
         pt.render(random_camera(np.random.default_rng(i)), 32, 32, 1).cpu()
     for name, ts in (("poisson", 1.0), ("saturated", 0.0)):
         for key, job in (("one_at_a_time", None), ("one_at_a_time_per_job", (ctx, built))):
-            lat, wall, (ext, shadow) = run_one_at_a_time(pts, trace, time_scale=ts, per_job=job)
+            lat, wall, (ext, shadow) = run_one_at_a_time(pts, trace, time_scale=ts, per_job=job, width=width)
             res[name][key] = {"p50_ms": round(_pct(lat, 50) * 1e3, 1), "p99_ms": round(_pct(lat, 99) * 1e3, 1),
                               "wall_s": round(wall, 3), "mpaths_s": round(paths / wall / 1e6, 2),
                               "mrays_s": round((ext + shadow) / wall / 1e6, 2)}
@@ -303,7 +342,8 @@ def bench_served(ctx, n_requests=60, rate=20.0, scale=1.0, wave_paths=0, infligh
         res[name]["throughput_ratio_per_job"] = round(res[name]["one_at_a_time_per_job"]["wall_s"] / s["wall_s"], 3)
     for pt in pts:
         pt.close()
-    res["server"] = {"wave_paths": wave_paths or (1 << 25), "inflight_waves": inflight_waves or 3, "graphs": graphs}
+    res["server"] = {"wave_paths": wave_paths or (1 << 25), "inflight_waves": inflight_waves or 3, "graphs": graphs,
+                     "width": width, "width_sample_ms": width_ms}
     res["requests"] = n_requests
     res["rate_per_s"] = rate
     res["paths"] = paths
