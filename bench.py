@@ -87,6 +87,35 @@ def profiled_traffic(a):
     return int(m.group(1)) if m else None
 
 
+# Algorithmic bytes of k_shade (pt.hip): per shaded queue entry it reads the path state (radiance,
+# two pending contributions, throughput, pixel and sample keys: 72 B), the path and request ids
+# (8 B), the hit (16 B) and ray (32 B), three vertex indices (12 B), three shading normals (36 B),
+# two positions for the edge tangent (24 B) and the triangle's material id (4 B) = 204 B, and
+# writes 64 B of path state; each continuation ray it emits is 40 B (ray, path id, scene) and each
+# shadow ray 40 B (ray, slot, scene). Records shared by many paths (materials, lights, requests)
+# are cache-resident and not counted.
+SHADE_READ_B, SHADE_WRITE_B, SHADE_RAY_B = 204, 64, 40
+
+
+def shade_roofline(pt, cam, W, H, spp, depth, torch):
+    """k_shade's HBM roofline for one frame: the frame rendered once more with stage timing (events
+    around every bounce's shading launch on the stream it runs on), algorithmic bytes from the
+    frame's ray counts (entries shaded = extension rays; continuations = extension rays - paths)."""
+    pt.set_timing(True)
+    pt.render(cam, W, H, spp, seed=1, max_depth=depth)
+    torch.cuda.synchronize()
+    st, tm = pt.stats(), pt.stage_times()
+    pt.set_timing(False)
+    ext, shadow, paths = st["extension_rays"], st["shadow_rays"], st["paths"]
+    b = ext * (SHADE_READ_B + SHADE_WRITE_B) + (ext - paths) * SHADE_RAY_B + shadow * SHADE_RAY_B
+    gbs = b / (tm["shade_ms"] * 1e-3) / 1e9 if tm["shade_ms"] > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": "k_shade", "bytes": int(b),
+            "entries": int(ext), "launches": tm["bounces"], "kernel_ms": round(tm["shade_ms"], 2),
+            "extend_ms": round(tm["extend_ms"], 2), "shadow_ms": round(tm["shadow_ms"], 2),
+            "note": "launch times on the render's two overlapping streams (shared GPU time included)"}
+
+
 def leg_pt(ctx, cpu=True):
     """Path tracer throughput on the Cornell box and on the 2M-triangle spheres scene."""
     import torch
@@ -120,6 +149,7 @@ def leg_pt(ctx, cpu=True):
                "ms": round(sec * 1e3, 2), "msamples_s": round(paths / sec / 1e6, 2),
                "mrays_s": round((ext + shadow) / reps / sec / 1e6, 2),
                "rays_per_path": round((ext + shadow) / reps / paths, 3), "build_s": round(build_s, 2)}
+        res["shade_roofline"] = shade_roofline(pt, cam, W, H, spp, depth, torch)
         if cpu:
             from oracle.oracle import OraclePT
             o = OraclePT(sc)

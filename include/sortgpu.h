@@ -344,6 +344,16 @@ int sortgpu_li(sortgpu_scene* scene, const sortgpu_ray* d_rays, const uint32_t* 
  * accelerator's error flags afterwards (SORTGPU_ERR_STATE when any is raised; out[] is filled). */
 int sortgpu_render_stats(sortgpu_scene* scene, uint64_t out[4]);
 
+/* Stage timing of sortgpu_render (no counterpart in SORT; measurement). With on = 1, later
+ * renders on the scene bracket each bounce's extension traversal, shading kernel and shadow
+ * traversal with timing events on the stream the bounce runs on (the first 1,024 bounces of a
+ * render); on = 0 stops it. sortgpu_render_stage_times (synchronous) gives for the last render
+ * the summed milliseconds of each stage, out_ms[0] extension traversal, [1] shading, [2] shadow
+ * rays, and *bounces the number of bounces timed (0 without timing). Waves on the render's two
+ * internal streams overlap, so a stage's time includes what it shared the GPU with. */
+int sortgpu_render_set_timing(sortgpu_scene* scene, int on);
+int sortgpu_render_stage_times(sortgpu_scene* scene, float out_ms[3], uint32_t* bounces);
+
 /* ---- Served form: continuous-batching render server ------------------------------------------
  * No counterpart in the reference, which renders one job per process (sort.cpp:32-60,
  * core/image_evaluation.cpp). The server keeps scenes, path state and films resident in HBM,
@@ -354,7 +364,7 @@ typedef struct sortgpu_server sortgpu_server;
 
 typedef struct sortgpu_server_opts {
     uint64_t hbm_budget;     /* bytes the server may hold: scenes + films + workspace (0 = 96 GiB) */
-    uint32_t wave_paths;     /* path slots per wave (0 = 33,554,432; 254 B of workspace each) */
+    uint32_t wave_paths;     /* path slots per wave (0 = 33,554,432; 260 B of workspace each) */
     uint32_t max_active;     /* requests admitted at once, 1..32 (0 = 32) */
     uint32_t inflight_waves; /* waves in flight at once, 1..4 (0 = 3), each on a stream and path
                                 workspace of its own, so their bounce loops overlap */

@@ -150,6 +150,8 @@ _SIGS = {
     "sortgpu_render_ray_dump": (_I, [_P, ctypes.POINTER(Camera), _U32, _U32, _U32, _U32, ctypes.c_uint64, _U32, _U32,
                                      _P, _P, _P, _P]),
     "sortgpu_render_stats": (_I, [_P, _P]),
+    "sortgpu_render_set_timing": (_I, [_P, ctypes.c_int]),
+    "sortgpu_render_stage_times": (_I, [_P, _P, _P]),
     "sortgpu_server_create": (_I, [_P, ctypes.POINTER(ServerOpts), ctypes.POINTER(_P)]),
     "sortgpu_server_destroy": (None, [_P]),
     "sortgpu_server_add_scene": (_I, [_P, _P, ctypes.POINTER(SceneDesc), ctypes.POINTER(_U32)]),

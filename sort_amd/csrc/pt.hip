@@ -587,6 +587,12 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
     hipError_t e;
     for (uint32_t b = 0; b < L.max_depth; b++) {
         const int cur = (int)(b & 1u), nxt = cur ^ 1;
+        hipEvent_t* tev = nullptr;  // this bounce's six stage events, when timed
+        if (L.stage_ev && *L.stage_used + 6 <= L.stage_cap) {
+            tev = L.stage_ev + *L.stage_used;
+            *L.stage_used += 6;
+        }
+        auto mark = [&](int k) -> hipError_t { return tev ? hipEventRecord(tev[k], s) : hipSuccess; };
         TraceArgs ta = L.trace;
         ta.rays = L.q.ray[cur];
         ta.n = npaths;
@@ -599,7 +605,9 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
             ta.width = L.tree_width;
             ta.max_stack = L.tree_max_stack;
         }
-        if ((e = launch_trace(ta, false, false, s, L.num_cus)) != hipSuccess) return e;
+        if ((e = mark(0)) != hipSuccess || (e = launch_trace(ta, false, false, s, L.num_cus)) != hipSuccess ||
+            (e = mark(1)) != hipSuccess)
+            return e;
         if (L.dump.rays) hipLaunchKernelGGL(k_dump, dim3(shade_blocks), dim3(kPtBlock), 0, s, b, cur, L.ps, L.q, L.dump);
         hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, L.q.count, nxt);
         const uint32_t* order = nullptr;
@@ -616,9 +624,11 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
             hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(256), 0, s, cur, L.q);
             order = L.q.sperm;
         }
+        if ((e = mark(2)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(kPtBlock), 0, s, L.scenes, L.wave->reqs, b, cur, L.ps,
                            L.q, order);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipGetLastError()) != hipSuccess || (e = mark(3)) != hipSuccess || (e = mark(4)) != hipSuccess)
+            return e;
         TraceArgs sa = ta;
         sa.rays = L.q.sray;
         sa.n = 2 * npaths;
@@ -645,6 +655,7 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
             if (L.trees) sa.ray_scene = L.q.ssc;
             if ((e = launch_trace(sa, true, false, s, L.num_cus)) != hipSuccess) return e;
         }
+        if ((e = mark(5)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_add_totals, dim3(1), dim3(64), 0, s, L.q.count, cur, L.q.totals);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

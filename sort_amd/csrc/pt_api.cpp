@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/sortgpu.h"
 #include "pt_host.h"
@@ -30,6 +31,12 @@ struct sortgpu_scene {
     ReqRing ring;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t last_paths = 0;
+    // stage timing: six events per bounce (sortgpu_render_set_timing); stage_used = events the
+    // last render recorded
+    static constexpr uint32_t kStageBounces = 1024;
+    std::vector<hipEvent_t> stage_ev;
+    bool timing = false;
+    uint32_t stage_used = 0;
 };
 
 namespace {
@@ -161,6 +168,7 @@ void sortgpu_scene_destroy(sortgpu_scene* sc) {
     sc->mem.free_all();
     if (sc->ev0) hipEventDestroy(sc->ev0);
     if (sc->ev1) hipEventDestroy(sc->ev1);
+    for (hipEvent_t ev : sc->stage_ev) hipEventDestroy(ev);
     delete sc;
     ctx_release(ctx);
 }
@@ -195,6 +203,7 @@ int render_impl(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, ui
     R.clamp = sc->host.clamp;
     hipError_t e = hipEventRecord(sc->ev0, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
+    sc->stage_used = 0;
     if ((e = pt_zero_f4(sc->film, npix, s)) != hipSuccess) return hip_fail(ctx, e, "zero film");
     for (int k = 0; k < sortgpu_scene::kStreams; k++)
         if (sc->ws[k].q.totals &&
@@ -221,6 +230,11 @@ int render_impl(sortgpu_scene* sc, const sortgpu_camera* cam, uint32_t width, ui
         L.paths_cap = npix * ns;
         L.pix_cap = npix;
         if (dump) L.dump = *dump;
+        if (sc->timing) {
+            L.stage_ev = sc->stage_ev.data();
+            L.stage_cap = (uint32_t)sc->stage_ev.size();
+            L.stage_used = &sc->stage_used;
+        }
         if ((e = pt_wave_paths(L, st)) != hipSuccess) return hip_fail(ctx, e, "render wave");
         // film updates in wave order, so the film is summed in sample order
         if (w > 0 && nstreams > 1 && (e = hipStreamWaitEvent(st, sc->ev_film, 0)) != hipSuccess)
@@ -328,6 +342,59 @@ int sortgpu_li(sortgpu_scene* sc, const sortgpu_ray* d_rays, const uint32_t* d_k
     if ((rc = sc->ring.done(ctx, s))) return rc;
     if ((e = hipEventRecord(sc->ev1, s)) != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
     sc->last_paths = n;
+    return SORTGPU_OK;
+}
+
+int sortgpu_render_set_timing(sortgpu_scene* sc, int on) {
+    if (!sc) return SORTGPU_ERR_ARG;
+    sortgpu_ctx* ctx = sc->accel->ctx;
+    int rc = ctx_bind(ctx);
+    if (rc) return rc;
+    if (on && sc->stage_ev.empty()) {
+        std::vector<hipEvent_t> evs;
+        try {
+            evs.reserve(6 * (size_t)sortgpu_scene::kStageBounces);
+        } catch (const std::bad_alloc&) {
+            set_error(ctx, "render timing: host allocation failed");
+            return SORTGPU_ERR_OOM;
+        }
+        for (uint32_t i = 0; i < 6 * sortgpu_scene::kStageBounces; i++) {
+            hipEvent_t ev = nullptr;
+            const hipError_t e = hipEventCreate(&ev);
+            if (e != hipSuccess) {
+                for (hipEvent_t x : evs) hipEventDestroy(x);
+                return hip_fail(ctx, e, "hipEventCreate(stage timing)");
+            }
+            evs.push_back(ev);
+        }
+        sc->stage_ev.swap(evs);
+    }
+    sc->timing = on != 0;
+    sc->stage_used = 0;
+    return SORTGPU_OK;
+}
+
+int sortgpu_render_stage_times(sortgpu_scene* sc, float out_ms[3], uint32_t* bounces) {
+    if (!sc || !out_ms) return SORTGPU_ERR_ARG;
+    sortgpu_ctx* ctx = sc->accel->ctx;
+    int rc = ctx_bind(ctx);
+    if (rc) return rc;
+    out_ms[0] = out_ms[1] = out_ms[2] = 0.0f;
+    if (bounces) *bounces = 0;
+    if (sc->stage_used == 0) return SORTGPU_OK;
+    hipError_t e = hipEventSynchronize(sc->ev1);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventSynchronize");
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (uint32_t b = 0; b + 6 <= sc->stage_used; b += 6) {
+        for (int k = 0; k < 3; k++) {
+            float ms = 0.0f;
+            if ((e = hipEventElapsedTime(&ms, sc->stage_ev[b + 2 * k], sc->stage_ev[b + 2 * k + 1])) != hipSuccess)
+                return hip_fail(ctx, e, "hipEventElapsedTime(stage)");
+            acc[k] += ms;
+        }
+    }
+    for (int k = 0; k < 3; k++) out_ms[k] = (float)acc[k];
+    if (bounces) *bounces = sc->stage_used / 6;
     return SORTGPU_OK;
 }
 

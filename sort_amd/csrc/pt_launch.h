@@ -40,6 +40,11 @@ struct PtLaunch {
     // each bounce's hits are shaded in spatial order when the wave holds at least this many paths
     // (0 = never)
     uint32_t sort_min;
+    // stage timing (or null): while *stage_used + 6 <= stage_cap, a bounce records six events from
+    // stage_ev[*stage_used] on (before / after its extension traversal, shading, shadow rays)
+    hipEvent_t* stage_ev;
+    uint32_t stage_cap;
+    uint32_t* stage_used;
 };
 
 // Bytes of path workspace per path slot (state + queues + hit order), used for HBM budgeting.

@@ -180,3 +180,15 @@ class PathTracing:
         out = np.zeros(4, np.uint64)
         check(lib().sortgpu_render_stats(self.handle, out.ctypes.data), self.accel.ctx.handle)
         return dict(zip(["extension_rays", "shadow_rays", "paths", "gpu_us"], [int(x) for x in out]))
+
+    def set_timing(self, on=True):
+        """Stage timing of later renders (events around each bounce's extension traversal,
+        shading kernel and shadow rays); read with stage_times()."""
+        check(lib().sortgpu_render_set_timing(self.handle, int(bool(on))), self.accel.ctx.handle)
+
+    def stage_times(self):
+        """Summed milliseconds per stage of the last timed render and the bounces timed."""
+        ms = np.zeros(3, np.float32)
+        n = ctypes.c_uint32(0)
+        check(lib().sortgpu_render_stage_times(self.handle, ms.ctypes.data, ctypes.byref(n)), self.accel.ctx.handle)
+        return {"extend_ms": float(ms[0]), "shade_ms": float(ms[1]), "shadow_ms": float(ms[2]), "bounces": int(n.value)}

@@ -221,3 +221,23 @@ def test_shade_order_changes_no_result(gpu_ctx, mat):
         set_tuning(gpu_ctx)
     assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
     assert np.array_equal(imgs[0].view(np.uint32), imgs[2].view(np.uint32))
+
+
+def test_stage_timing_changes_no_result(gpu_ctx):
+    """Stage timing (events around each bounce's traversal, shading and shadow rays) leaves the
+    image bitwise unchanged and accounts for every bounce of the frame's waves."""
+    sc = scenes.cornell_sphere()
+    acc, pt = _gpu(gpu_ctx, sc, 5)
+    cam = I.camera(**sc["camera"])
+    a = pt.render(cam, 64, 64, 8, seed=3).cpu().numpy()
+    assert pt.stage_times()["bounces"] == 0
+    pt.set_timing(True)
+    b = pt.render(cam, 64, 64, 8, seed=3).cpu().numpy()
+    t = pt.stage_times()
+    print(f"[pt] stage times {t}", flush=True)
+    pt.set_timing(False)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert t["bounces"] >= 5 and t["bounces"] % 5 == 0
+    assert t["extend_ms"] > 0 and t["shade_ms"] > 0 and t["shadow_ms"] > 0
+    total_ms = pt.stats()["gpu_us"] / 1e3
+    assert t["extend_ms"] + t["shade_ms"] + t["shadow_ms"] <= 2 * total_ms + 1.0
