@@ -381,8 +381,10 @@ def probe_lib():
 def memory_pipeline(n, st, mean_ms, value, stream, width):
     """The ceiling the traversal runs against: each lane reads its own records (an 8-wide node as
     8 consecutive 16-byte chunks, 14 when it has more than four children; a 4-wide node as 7; a
-    triangle record as 3), so a launch is bound by the rate at which compute units return such
-    chunks to lanes. The probe library replays those access patterns on an L2-resident buffer
+    leaf record of two triangles, or one quad, as 5), so a launch is bound by the rate at which
+    compute units return such chunks to lanes. Leaf records per ray are estimated as half the
+    triangles tested plus a quarter of the leaves visited (a leaf with an odd count ends on a
+    one-triangle record) plus the quads. The probe library replays those access patterns on an L2-resident buffer
     (the traversal's upper levels live in L2; its own L2 hit rate is lower, so this is an upper
     bound) and gives chunks/s; the ceiling is the launch time those rates allow for this launch's
     chunk counts."""
@@ -396,7 +398,7 @@ def memory_pipeline(n, st, mean_ms, value, stream, width):
     out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
     iters = 256
     rates = {}
-    for k in sorted({node_chunks, narrow_chunks, 3}):
+    for k in sorted({node_chunks, narrow_chunks, 5, 3}):
         def run():
             rc = p.chunk_probe(buf.data_ptr(), 8192, k, iters, blocks, out.data_ptr(), stream.cuda_stream)
             if rc != 0:
@@ -405,10 +407,11 @@ def memory_pipeline(n, st, mean_ms, value, stream, width):
         rates[k] = blocks * 256 * iters * k / (ms * 1e-3) / 1e9  # G chunks/s
     wide = st.get("wide_nodes", 0) if width == 8 else st["inner_nodes"]
     narrow = st["inner_nodes"] - wide
+    records = st["tris"] / 2 + st["leaves"] / 4 + st["quads"]
     per_ray = {"wide_node": node_chunks * wide / n, "node": narrow_chunks * narrow / n,
-               "tri": 3 * (st["tris"] + st["quads"]) / n, "ray": 2.0}
+               "leaf_record": 5 * records / n, "ray": 2.0}
     ns_per_ray = (per_ray["wide_node"] / rates[node_chunks] + per_ray["node"] / rates[narrow_chunks] +
-                  (per_ray["tri"] + per_ray["ray"]) / rates[3])
+                  per_ray["leaf_record"] / rates[5] + per_ray["ray"] / rates[3])
     ceiling = 1e3 / ns_per_ray  # Mrays/s
     chunks = sum(per_ray.values())
     return {"bound": "vector-memory return rate (16-byte chunks to lanes)",

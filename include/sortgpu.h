@@ -83,7 +83,8 @@ const char* sortgpu_last_error(const sortgpu_ctx* ctx);
  * pos: nverts x 3 floats; idx: ntris x 3 vertex indices; quads: nquads records. The SAH
  * 8-wide (or 4-wide) tree is built on the host, flattened into the device layout and
  * uploaded. Input arrays are copied; the caller keeps ownership. An empty scene yields a
- * valid handle whose queries all miss (m_isValid = false, fast_bvh.hpp:112-114). */
+ * valid handle whose queries all miss (m_isValid = false, fast_bvh.hpp:112-114). At most
+ * 2^30 - 1 primitives (triangles + quads) per tree (SORTGPU_ERR_ARG beyond). */
 int sortgpu_accel_build(sortgpu_ctx* ctx, const float* pos, uint32_t nverts, const uint32_t* idx,
                         uint32_t ntris, const sortgpu_quad* quads, uint32_t nquads,
                         const sortgpu_build_opts* opts, sortgpu_accel** out);

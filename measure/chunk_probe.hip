@@ -1,8 +1,8 @@
 // Vector-memory return-rate probe: measurement code of the benchmark (measure/libchunk_probe.so),
 // not part of the library or of its drop-in boundary.
 //
-// A traversal lane reads its own record: 14 consecutive 16-byte chunks of a 256-byte node, or 3
-// of a 48-byte triangle record, at addresses no other lane shares. What bounds such a kernel is the
+// A traversal lane reads its own record: 14 consecutive 16-byte chunks of a 256-byte node, or 5
+// of an 80-byte leaf record (two triangles), at addresses no other lane shares. What bounds such a kernel is the
 // rate at which a compute unit returns 16-byte chunks to lanes, not HBM bandwidth. This kernel
 // reproduces that access pattern without the traversal: every lane reads `chunks` consecutive
 // chunks of a pseudo-random 256-byte record, `iters` times, and the host divides chunks by time.
@@ -50,7 +50,7 @@ hipError_t launch(const void* buf, uint32_t n_rec, int iters, int blocks, float*
 }  // namespace
 
 // Launches `blocks` workgroups of 256 lanes on `stream` (a hipStream_t, NULL = default stream).
-// Every lane reads, `iters` times, `chunks` (1, 3, 7, 8, 14 or 16) consecutive 16-byte chunks of a
+// Every lane reads, `iters` times, `chunks` (1, 3, 5, 7, 8, 14 or 16) consecutive 16-byte chunks of a
 // pseudo-random 256-byte record of `buf` (a device buffer of n_rec * 256 bytes), all chunks of a
 // record issued back to back, and writes one float to out[lane] (blocks * 256 floats). Returns 0
 // or a hipError_t code (other chunk counts are refused); asynchronous.
@@ -64,6 +64,7 @@ extern "C" int chunk_probe(const void* buf, uint32_t n_rec, int chunks, int iter
     switch (chunks) {
         case 1: return (int)launch<1>(buf, n_rec, iters, blocks, out, s);
         case 3: return (int)launch<3>(buf, n_rec, iters, blocks, out, s);
+        case 5: return (int)launch<5>(buf, n_rec, iters, blocks, out, s);
         case 7: return (int)launch<7>(buf, n_rec, iters, blocks, out, s);
         case 8: return (int)launch<8>(buf, n_rec, iters, blocks, out, s);
         case 14: return (int)launch<14>(buf, n_rec, iters, blocks, out, s);

@@ -79,6 +79,7 @@ struct BPrim {
 
 struct BNode {
     uint32_t off = 0, cnt = 0, child_cnt = 0;
+    uint32_t rec = 0xFFFFFFFFu;  // flattened leaf: index of its first leaf record
     std::unique_ptr<BNode> child[8];
     BNode(uint32_t o, uint32_t c) : off(o), cnt(c) {}
 };
@@ -280,53 +281,73 @@ struct Flattener {
     std::vector<NodeRec<W>> nodes;
     std::vector<PrimRec> prims;
     std::vector<uint32_t> dump;
+    uint32_t next_rec = 0;  // leaf records assigned so far
 
     Flattener(const Builder& b, const BuildInput& i) : B(b), in(i) {}
 
-    uint32_t leaf_ref(const BNode* n) { return REF_LEAF | n->off; }
+    // The leaf's record range, assigned when its reference is first written (its parent's
+    // record is filled before the leaf's records are), in that order.
+    uint32_t leaf_ref(BNode* n) {
+        if (n->rec == 0xFFFFFFFFu) {
+            uint32_t tris = 0;
+            for (uint32_t i = n->off; i < n->off + n->cnt; i++) tris += B.P[i].id < in.ntris ? 1u : 0u;
+            n->rec = next_rec;
+            next_rec += (tris + 1) / 2 + (n->cnt - tris);
+        }
+        return REF_LEAF | n->rec;
+    }
 
-    // Leaf records: triangles (build order) then quads, last record flagged. The triangles form
-    // groups of W in that order, the last one possibly partial (the reference packs a leaf's
-    // triangles into Simd_Triangle groups this way, fast_bvh.hpp:226-251); the last triangle of
-    // each group is flagged PR_GEND.
-    void emit_leaf(const BNode* n) {
-        uint32_t w = n->off;
+    // Leaf records: triangles (build order) two per record, then quads one per record, the last
+    // record flagged. The triangles form groups of W in that order, the last one possibly partial
+    // (the reference packs a leaf's triangles into Simd_Triangle groups this way,
+    // fast_bvh.hpp:226-251); the last triangle of each group carries PR_GEND.
+    void emit_leaf(BNode* n) {
+        const uint32_t first = leaf_ref(n) & ~REF_LEAF;
+        uint32_t w = first;
         uint32_t j = 0;  // triangles emitted so far in this leaf
+        uint32_t tris = 0;
+        for (uint32_t i = n->off; i < n->off + n->cnt; i++) tris += B.P[i].id < in.ntris ? 1u : 0u;
         for (uint32_t i = n->off; i < n->off + n->cnt; i++) {
             const uint32_t id = B.P[i].id;
             if (id >= in.ntris) continue;
-            if (j > 0 && j % W == 0) prims[w - 1].flags |= PR_GEND;
-            j++;
-            PrimRec& r = prims[w++];
+            PrimRec& r = prims[w];
+            Tri& x = (j & 1u) ? r.b : r.a;
             const uint32_t* t = in.idx + 3 * (size_t)id;
             const float* a = in.pos + 3 * (size_t)t[0];
             const float* b = in.pos + 3 * (size_t)t[1];
             const float* c = in.pos + 3 * (size_t)t[2];
-            r.p0x = a[0]; r.p0y = a[1]; r.p0z = a[2];
-            r.p1x = b[0]; r.p1y = b[1]; r.p1z = b[2];
-            r.p2x = c[0]; r.p2y = c[1]; r.p2z = c[2];
-            r.prim = id;
-            r.flags = 0;
-            r.aux = 0;
+            x.p0x = a[0]; x.p0y = a[1]; x.p0z = a[2];
+            x.p1x = b[0]; x.p1y = b[1]; x.p1z = b[2];
+            x.p2x = c[0]; x.p2y = c[1]; x.p2z = c[2];
+            const uint32_t gend = ((j + 1) % W == 0 || j + 1 == tris) ? PR_GEND : 0u;
+            if (j & 1u) {
+                r.id_b = id | gend;
+                w++;
+            } else {
+                r.id_a = id | gend;
+                r.id_b = PR_NONE;
+            }
+            j++;
         }
-        if (j > 0) prims[w - 1].flags |= PR_GEND;
+        if (j & 1u) w++;
         for (uint32_t i = n->off; i < n->off + n->cnt; i++) {
             const uint32_t id = B.P[i].id;
             if (id < in.ntris) continue;
             PrimRec& r = prims[w++];
             std::memset(&r, 0, sizeof(r));
-            r.prim = id;
-            r.flags = PR_QUAD;
-            r.aux = id - in.ntris;
+            const uint32_t q = id - in.ntris;
+            std::memcpy(&r.a.p0x, &q, sizeof(q));
+            r.id_a = id | PR_FLAG;
+            r.id_b = PR_NONE;
         }
-        prims[n->off + n->cnt - 1].flags |= PR_LAST;
+        prims[w - 1].id_b |= PR_FLAG;
     }
 
     // Children of a node get consecutive node indices.
     void emit_inner(const BNode* n, uint32_t self) {
         uint32_t child_idx[8];
         for (uint32_t j = 0; j < n->child_cnt; j++) {
-            const BNode* ch = n->child[j].get();
+            BNode* ch = n->child[j].get();
             if (ch->child_cnt) {
                 child_idx[j] = (uint32_t)nodes.size();
                 nodes.emplace_back();
@@ -344,12 +365,12 @@ struct Flattener {
             if ((uint32_t)j >= n->child_cnt) {
                 r.child[j] = REF_EMPTY;
             } else {
-                const BNode* ch = n->child[j].get();
+                BNode* ch = n->child[j].get();
                 r.child[j] = ch->child_cnt ? child_idx[j] : leaf_ref(ch);
             }
         }
         for (uint32_t j = 0; j < n->child_cnt; j++) {
-            const BNode* ch = n->child[j].get();
+            BNode* ch = n->child[j].get();
             if (ch->child_cnt) emit_inner(ch, child_idx[j]);
             else emit_leaf(ch);
         }
@@ -362,17 +383,21 @@ struct Flattener {
         for (uint32_t j = 0; j < n->child_cnt; j++) dump_node(n->child[j].get());
     }
 
-    uint32_t run(const BNode* root, uint32_t nprims) {
+    uint32_t run(BNode* root, uint32_t nprims) {
+        // at most one record per primitive, plus a zero record after the last
         prims.assign(nprims + 1, PrimRec{});
         dump_node(root);
+        uint32_t ref = 0;
         if (!root->child_cnt) {
             emit_leaf(root);
-            return leaf_ref(root);
+            ref = leaf_ref(root);
+        } else {
+            nodes.reserve(B.n_inner.load());
+            nodes.emplace_back();
+            emit_inner(root, 0);
         }
-        nodes.reserve(B.n_inner.load());
-        nodes.emplace_back();
-        emit_inner(root, 0);
-        return 0;
+        prims.resize((size_t)next_rec + 1);
+        return ref;
     }
 };
 
@@ -393,8 +418,8 @@ bool build_tree(const BuildInput& in, BuiltTree& out, std::string& err) {
         return false;
     }
     const uint64_t n64 = (uint64_t)in.ntris + in.nquads;
-    if (n64 >= REF_LEAF) {
-        err = "too many primitives (limit 2^31-1)";
+    if (n64 > PR_ID) {  // primitive ids share their word with two record flags
+        err = "too many primitives (limit 2^30-1)";
         return false;
     }
     for (uint64_t i = 0; i < 3ull * in.ntris; i++) {

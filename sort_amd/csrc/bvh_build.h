@@ -47,7 +47,7 @@ struct BuiltTree {
     std::vector<uint32_t> prim_order;       // m_bvhpri order
     std::vector<uint32_t> dump;             // pre-order (child_cnt, pri_offset, pri_cnt)
     std::vector<unsigned char> node_bytes;  // n_inner NodeRec<width>
-    std::vector<PrimRec> prims;             // nprims records (+1 zero pad)
+    std::vector<PrimRec> prims;             // leaf records (+1 zero record)
     std::vector<QuadRec> quads;
 };
 

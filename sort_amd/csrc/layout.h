@@ -11,10 +11,12 @@
 //   bit31 = 0             inner child: node index
 //   bit31 = 1             leaf: index of its first primitive record
 //
-// Primitive records: one 48-byte record per primitive, in tree order. A leaf's records are
-// contiguous: its triangles in build order, then its quads (the reference tests a leaf's
-// Triangle8 groups first and its other_list after, fast_bvh.hpp:349-406). The last record of a
-// leaf carries PR_LAST.
+// Leaf records: 80 bytes (five 16-byte loads), in tree order. A leaf's records are contiguous:
+// its triangles in build order, two per record (A then B; the last record of an odd count holds
+// one), then its quads, one per record (the reference tests a leaf's Triangle8 groups first and
+// its other_list after, fast_bvh.hpp:349-406). A leaf walk loads one record per step, so a pair
+// record halves the steps of a triangle walk. Flags live in the top bits of the two primitive
+// ids, which limits a tree to 2^30 - 1 primitives.
 #pragma once
 #include <stdint.h>
 
@@ -23,9 +25,11 @@ namespace sortgpu {
 constexpr uint32_t REF_EMPTY = 0xFFFFFFFFu;
 constexpr uint32_t REF_LEAF = 0x80000000u;
 
-constexpr uint32_t PR_LAST = 1u;  // last record of the leaf
-constexpr uint32_t PR_QUAD = 2u;  // record is a quad; `prim` field holds the quad index
-constexpr uint32_t PR_GEND = 4u;  // last triangle of a group of W (the reference's Simd_Triangle)
+constexpr uint32_t PR_ID = 0x3FFFFFFFu;    // primitive id bits of id_a / id_b
+constexpr uint32_t PR_NONE = 0x3FFFFFFFu;  // id_b: the record holds triangle A only
+constexpr uint32_t PR_FLAG = 0x80000000u;  // id_a: the record is a quad; id_b: last record of the leaf
+constexpr uint32_t PR_GEND = 0x40000000u;  // id_a / id_b: that triangle ends a group of W (the
+                                           // reference's Simd_Triangle)
 
 template <int W> struct alignas(W == 8 ? 256 : 128) NodeRec {
     float lo_x[W], hi_x[W], lo_y[W], hi_y[W], lo_z[W], hi_z[W];
@@ -35,15 +39,17 @@ template <int W> struct alignas(W == 8 ? 256 : 128) NodeRec {
 static_assert(sizeof(NodeRec<8>) == 256, "Node8 must be 256 B");
 static_assert(sizeof(NodeRec<4>) == 128, "Node4 must be 128 B");
 
-struct alignas(16) PrimRec {
-    float p0x, p0y, p0z, p1x;
-    float p1y, p1z, p2x, p2y;
-    float p2z;
-    uint32_t prim;   // primitive id (triangle id, or ntris + quad index for quads)
-    uint32_t flags;  // PR_LAST | PR_QUAD
-    uint32_t aux;    // quad index for PR_QUAD records
+// A triangle's vertices as the watertight test reads them.
+struct Tri {
+    float p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z;
 };
-static_assert(sizeof(PrimRec) == 48, "PrimRec must be 48 B");
+
+struct alignas(16) PrimRec {
+    Tri a, b;        // triangles A and B; a quad record keeps its quad index in a.p0x's bits
+    uint32_t id_a;   // A's primitive id | PR_GEND; a quad: its primitive id | PR_FLAG
+    uint32_t id_b;   // B's primitive id (PR_NONE: no B) | PR_GEND; | PR_FLAG on the leaf's last record
+};
+static_assert(sizeof(PrimRec) == 80, "PrimRec must be 80 B");
 
 // Quad light record: inverse transform (world -> light), half sizes, primitive id.
 struct alignas(16) QuadRec {

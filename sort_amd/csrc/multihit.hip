@@ -51,42 +51,47 @@ __global__ __launch_bounds__(kMultiBlock) void k_multi(MultiArgs a) {
         const uint32_t ref = e.x;
         if (maxt < __uint_as_float(e.y)) continue;
         if (ref & REF_LEAF) {
-            // triangle groups of W records (PR_GEND ends one): every hit of a group is judged
-            // against maxt as it was before the group; quads are not tested
+            // triangle groups of W (PR_GEND ends one): every hit of a group is judged against maxt
+            // as it was before the group; quads are not tested
             uint32_t li = ref & ~REF_LEAF;
             bool gstart = true;
             float gmaxt = maxt;
+            auto tri = [&](const Tri& x, uint32_t id) {
+                if (gstart) {
+                    gmaxt = maxt;
+                    gstart = false;
+                }
+                const uint32_t prim = id & PR_ID;
+                float t, u, v;
+                if (tri_test(r, x, t, u, v) && t < gmaxt && a.tri_material[prim] == mat) {
+                    int slot;
+                    const bool full = cnt >= 4u;
+                    if (!full) {
+                        slot = (int)cnt++;
+                    } else {
+                        // the first record with the largest t (strictly above 0)
+                        slot = -1;
+                        float mt = 0.0f;
+                        if (mt < rt0) { mt = rt0; slot = 0; }
+                        if (mt < rt1) { mt = rt1; slot = 1; }
+                        if (mt < rt2) { mt = rt2; slot = 2; }
+                        if (mt < rt3) { mt = rt3; slot = 3; }
+                    }
+                    if (slot == 0) { rt0 = t; rp0 = prim; ru0 = u; rv0 = v; }
+                    if (slot == 1) { rt1 = t; rp1 = prim; ru1 = u; rv1 = v; }
+                    if (slot == 2) { rt2 = t; rp2 = prim; ru2 = u; rv2 = v; }
+                    if (slot == 3) { rt3 = t; rp3 = prim; ru3 = u; rv3 = v; }
+                    if (full) maxt = smax(smax(smax(rt0, rt1), rt2), rt3);
+                }
+                if (id & PR_GEND) gstart = true;
+            };
             while (true) {
                 const PrimRec pr = load_prim(prims, li);
-                if (!(pr.flags & PR_QUAD)) {
-                    if (gstart) {
-                        gmaxt = maxt;
-                        gstart = false;
-                    }
-                    float t, u, v;
-                    if (tri_test(r, pr, t, u, v) && t < gmaxt && a.tri_material[pr.prim] == mat) {
-                        int slot;
-                        const bool full = cnt >= 4u;
-                        if (!full) {
-                            slot = (int)cnt++;
-                        } else {
-                            // the first record with the largest t (strictly above 0)
-                            slot = -1;
-                            float mt = 0.0f;
-                            if (mt < rt0) { mt = rt0; slot = 0; }
-                            if (mt < rt1) { mt = rt1; slot = 1; }
-                            if (mt < rt2) { mt = rt2; slot = 2; }
-                            if (mt < rt3) { mt = rt3; slot = 3; }
-                        }
-                        if (slot == 0) { rt0 = t; rp0 = pr.prim; ru0 = u; rv0 = v; }
-                        if (slot == 1) { rt1 = t; rp1 = pr.prim; ru1 = u; rv1 = v; }
-                        if (slot == 2) { rt2 = t; rp2 = pr.prim; ru2 = u; rv2 = v; }
-                        if (slot == 3) { rt3 = t; rp3 = pr.prim; ru3 = u; rv3 = v; }
-                        if (full) maxt = smax(smax(smax(rt0, rt1), rt2), rt3);
-                    }
-                    if (pr.flags & PR_GEND) gstart = true;
+                if (!(pr.id_a & PR_FLAG)) {
+                    tri(pr.a, pr.id_a);
+                    if ((pr.id_b & PR_ID) != PR_NONE) tri(pr.b, pr.id_b);
                 }
-                if (pr.flags & PR_LAST) break;
+                if (pr.id_b & PR_FLAG) break;
                 ++li;
             }
             continue;

@@ -41,13 +41,42 @@ constexpr int kBlock = 256;
 constexpr int kDefaultLds = 16;
 constexpr uint32_t kDefaultLeafBatch = 32;
 
-// One primitive record of a lane's leaf: `pr` is tested while the leaf's next record (if any) is
-// loaded into `nxt`. Closest hit keeps the first record with the smallest t in (tmin, tmax]
-// (strict t < bt for triangles; a quad at the current t replaces it); any hit ends the ray.
-// Transparent-shadow walk (TRANSP): a closest-hit walk that stops, blocked, when a group of W
-// triangles (PR_GEND ends one) or a quad makes the closest hit and that primitive's material has no
-// transparency (fast_bvh.hpp:352-373 and 394-405, the ENABLE_TRANSPARENT_SHADOW branches: only the
-// group's nearest hit is checked). `gup`: the current group has made the closest hit.
+// One triangle of a leaf record, in record order. Closest hit keeps the first triangle with the
+// smallest t in (tmin, tmax] (strict t < bt); any hit ends the ray at a hit. Transparent-shadow
+// walk (TRANSP): a closest-hit walk that stops, blocked, when a group of W triangles (PR_GEND
+// ends one) makes the closest hit and that primitive's material has no transparency
+// (fast_bvh.hpp:352-373, the ENABLE_TRANSPARENT_SHADOW branch: only the group's nearest hit is
+// checked). `gup`: the current group has made the closest hit. Returns true when the walk stops.
+template <bool ANY, bool STATS, bool TRANSP>
+__device__ __forceinline__ bool tri_step(const RayK& r, const Tri& tri, uint32_t id, const float* __restrict__ transp,
+                                         float& bt, uint32_t& bp, float& bu, float& bv, bool& gup,
+                                         uint32_t& n_tris) {
+    if (STATS) n_tris++;
+    float t, u, v;
+    bool hit = false;
+    if (tri_test(r, tri, t, u, v) && t < bt) {
+        bt = t;
+        bp = id & PR_ID;
+        bu = u;
+        bv = v;
+        hit = true;
+    }
+    if (TRANSP) {
+        gup |= hit;
+        if (id & PR_GEND) {
+            const bool blocked = gup && (!transp || transp[4 * (size_t)bp + 3] == 0.0f);
+            gup = false;
+            return blocked;
+        }
+        return false;
+    }
+    return ANY && hit;
+}
+
+// One leaf record of a lane's leaf: `pr` is tested while the leaf's next record (if any) is
+// loaded into `nxt`: its triangles A then B, or its quad (a quad at the current closest t
+// replaces it; in the transparent-shadow walk a quad that makes the closest hit blocks when its
+// material has no transparency, fast_bvh.hpp:394-405).
 template <bool ANY, bool STATS, bool TRANSP>
 __device__ __forceinline__ void leaf_step(const RayK& r, const PrimRec* __restrict__ prims,
                                           const QuadRec* __restrict__ quads, const float* __restrict__ transp,
@@ -55,45 +84,29 @@ __device__ __forceinline__ void leaf_step(const RayK& r, const PrimRec* __restri
                                           uint32_t& bp, float& bu, float& bv, bool& occluded, bool& gup,
                                           uint32_t& n_tris, uint32_t& n_quads) {
     if (li == 0xffffffffu) return;
-    if (!(pr.flags & PR_LAST)) nxt = load_prim(prims, li + 1);
-    float t, u, v;
-    bool hit = false;
-    if (pr.flags & PR_QUAD) {
+    const bool last = (pr.id_b & PR_FLAG) != 0;
+    if (!last) nxt = load_prim(prims, li + 1);
+    bool stop = false;
+    if (pr.id_a & PR_FLAG) {
         if (STATS) n_quads++;
-        if (quad_test(r, quads[pr.aux], bt, t)) {
+        float t;
+        if (quad_test(r, quads[__float_as_uint(pr.a.p0x)], bt, t)) {
             bt = t;
-            bp = pr.prim;
+            bp = pr.id_a & PR_ID;
             bu = bv = 0.f;
-            hit = true;
+            stop = ANY || (TRANSP && (!transp || transp[4 * (size_t)bp + 3] == 0.0f));
         }
     } else {
-        if (STATS) n_tris++;
-        if (tri_test(r, pr, t, u, v) && t < bt) {
-            bt = t;
-            bp = pr.prim;
-            bu = u;
-            bv = v;
-            hit = true;
-        }
+        stop = tri_step<ANY, STATS, TRANSP>(r, pr.a, pr.id_a, transp, bt, bp, bu, bv, gup, n_tris);
+        if (!stop && (pr.id_b & PR_ID) != PR_NONE)
+            stop = tri_step<ANY, STATS, TRANSP>(r, pr.b, pr.id_b, transp, bt, bp, bu, bv, gup, n_tris);
     }
-    bool blocked = false;
-    if (TRANSP) {
-        if (pr.flags & PR_QUAD) {
-            blocked = hit && (!transp || transp[4 * (size_t)bp + 3] == 0.0f);
-        } else {
-            gup |= hit;
-            if (pr.flags & PR_GEND) {
-                blocked = gup && (!transp || transp[4 * (size_t)bp + 3] == 0.0f);
-                gup = false;
-            }
-        }
-    }
-    if ((ANY && hit) || blocked) {
+    if (stop) {
         occluded = true;
         li = 0xffffffffu;
         sp = 0;
     } else {
-        li = (pr.flags & PR_LAST) ? 0xffffffffu : li + 1;
+        li = last ? 0xffffffffu : li + 1;
     }
 }
 

@@ -106,7 +106,7 @@ __device__ __forceinline__ float root_slab(const RayK& r, const float* bb) {
 
 // Watertight shear test of one triangle (one lane of intersectTriangleInner_SIMD). Returns true
 // with t, u, v when tmin < t <= tmax.
-__device__ __forceinline__ bool tri_test(const RayK& r, const PrimRec& p, float& t_out, float& u_out,
+__device__ __forceinline__ bool tri_test(const RayK& r, const Tri& p, float& t_out, float& u_out,
                                          float& v_out) {
     float p0x = sel3(p.p0x, p.p0y, p.p0z, r.kx) - r.okx;
     float p0y = sel3(p.p0x, p.p0y, p.p0z, r.ky) - r.oky;
@@ -172,16 +172,18 @@ __device__ __forceinline__ bool quad_test(const RayK& r, const QuadRec& q, float
     return true;
 }
 
+// One leaf record: five 16-byte loads.
 __device__ __forceinline__ PrimRec load_prim(const PrimRec* __restrict__ base, uint32_t i) {
     const float4* p = reinterpret_cast<const float4*>(base + i);
-    const float4 a = p[0], b = p[1], c = p[2];
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
     PrimRec r;
-    r.p0x = a.x; r.p0y = a.y; r.p0z = a.z; r.p1x = a.w;
-    r.p1y = b.x; r.p1z = b.y; r.p2x = b.z; r.p2y = b.w;
-    r.p2z = c.x;
-    r.prim = __float_as_uint(c.y);
-    r.flags = __float_as_uint(c.z);
-    r.aux = __float_as_uint(c.w);
+    r.a.p0x = a.x; r.a.p0y = a.y; r.a.p0z = a.z; r.a.p1x = a.w;
+    r.a.p1y = b.x; r.a.p1z = b.y; r.a.p2x = b.z; r.a.p2y = b.w;
+    r.a.p2z = c.x; r.b.p0x = c.y; r.b.p0y = c.z; r.b.p0z = c.w;
+    r.b.p1x = d.x; r.b.p1y = d.y; r.b.p1z = d.z; r.b.p2x = d.w;
+    r.b.p2y = e.x; r.b.p2z = e.y;
+    r.id_a = __float_as_uint(e.z);
+    r.id_b = __float_as_uint(e.w);
     return r;
 }
 
