@@ -52,7 +52,8 @@ __global__ __launch_bounds__(kMultiBlock) void k_multi(MultiArgs a) {
         if (maxt < __uint_as_float(e.y)) continue;
         if (ref & REF_LEAF) {
             // triangle groups of W (PR_GEND ends one): every hit of a group is judged against maxt
-            // as it was before the group; quads are not tested
+            // as it was before the group; quads are not tested. The leaf's next record loads while
+            // the current one is tested.
             uint32_t li = ref & ~REF_LEAF;
             bool gstart = true;
             float gmaxt = maxt;
@@ -85,13 +86,17 @@ __global__ __launch_bounds__(kMultiBlock) void k_multi(MultiArgs a) {
                 }
                 if (id & PR_GEND) gstart = true;
             };
+            PrimRec pr = load_prim(prims, li);
             while (true) {
-                const PrimRec pr = load_prim(prims, li);
+                const bool last = (pr.id_b & PR_FLAG) != 0;
+                PrimRec nxt = pr;
+                if (!last) nxt = load_prim(prims, li + 1);
                 if (!(pr.id_a & PR_FLAG)) {
                     tri(pr.a, pr.id_a);
                     if ((pr.id_b & PR_ID) != PR_NONE) tri(pr.b, pr.id_b);
                 }
-                if (pr.id_b & PR_FLAG) break;
+                if (last) break;
+                pr = nxt;
                 ++li;
             }
             continue;
