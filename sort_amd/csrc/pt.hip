@@ -273,37 +273,54 @@ __global__ __launch_bounds__(256) void k_sort_count(int cur, const TreeView* tre
     for (uint32_t j = threadIdx.x; j < kSortBuckets; j += 256) q.shist[blockIdx.x * kSortBuckets + j] = h[j];
 }
 
-// Per bucket: exclusive offsets of the slices within the bucket, and the bucket's total.
+// Per bucket: exclusive offsets of the slices within the bucket, and the bucket's total. A
+// workgroup takes 16 buckets; each of its lanes sums 32 consecutive slices of one bucket (the 16
+// lanes of a slice row read 64 contiguous bytes), the 16 partial sums of a bucket are scanned in
+// LDS, and each lane writes the running offsets of its slices.
+constexpr uint32_t kScanBuckets = 16, kScanGroups = 256 / kScanBuckets, kScanRun = kSortBlocks / kScanGroups;
+static_assert(kSortBuckets % kScanBuckets == 0 && kSortBlocks % kScanGroups == 0, "scan tiling");
 __global__ __launch_bounds__(256) void k_sort_scan_slices(Queues q) {
-    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= kSortBuckets) return;
+    __shared__ uint32_t part[kScanGroups][kScanBuckets];
+    const uint32_t jb = threadIdx.x % kScanBuckets, g = threadIdx.x / kScanBuckets;
+    const uint32_t j = blockIdx.x * kScanBuckets + jb;
+    uint32_t* col = q.shist + (size_t)g * kScanRun * kSortBuckets + j;
+    uint32_t v[kScanRun], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanRun; k++) v[k] = col[(size_t)k * kSortBuckets];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanRun; k++) sum += v[k];
+    part[g][jb] = sum;
+    __syncthreads();
     uint32_t run = 0;
-    for (uint32_t b = 0; b < kSortBlocks; b++) {
-        const uint32_t v = q.shist[b * kSortBuckets + j];
-        q.shist[b * kSortBuckets + j] = run;
-        run += v;
+    for (uint32_t k = 0; k < g; k++) run += part[k][jb];
+    if (g == kScanGroups - 1) q.sbase[j] = run + sum;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanRun; k++) {
+        col[(size_t)k * kSortBuckets] = run;
+        run += v[k];
     }
-    q.sbase[j] = run;
 }
 
-// Exclusive scan of the bucket totals (one workgroup of 1024 lanes, 4 buckets each).
+// Exclusive scan of the bucket totals (one workgroup of 1024 lanes, 4 buckets each): a scan
+// within each 64-lane wave by lane shuffles, then over the 16 wave totals.
 __global__ __launch_bounds__(1024) void k_sort_scan_buckets(Queues q) {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
     uint32_t v[4], sum = 0;
     for (int k = 0; k < 4; k++) {
         v[k] = q.sbase[4 * t + k];
         sum += v[k];
     }
-    part[t] = sum;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-        const uint32_t x = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += x;
-        __syncthreads();
+    uint32_t incl = sum;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t x = __shfl_up(incl, d);
+        if (lane >= d) incl += x;
     }
-    uint32_t run = part[t] - sum;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (uint32_t k = 0; k < w; k++) run += wsum[k];
     for (int k = 0; k < 4; k++) {
         q.sbase[4 * t + k] = run;
         run += v[k];
@@ -619,7 +636,7 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
                 box.inv[k] = w > 0.0f ? 1.0f / w : 0.0f;
             }
             hipLaunchKernelGGL(k_sort_count, dim3(kSortBlocks), dim3(256), 0, s, cur, L.trees, box, L.q);
-            hipLaunchKernelGGL(k_sort_scan_slices, dim3(kSortBuckets / 256), dim3(256), 0, s, L.q);
+            hipLaunchKernelGGL(k_sort_scan_slices, dim3(kSortBuckets / kScanBuckets), dim3(256), 0, s, L.q);
             hipLaunchKernelGGL(k_sort_scan_buckets, dim3(1), dim3(1024), 0, s, L.q);
             hipLaunchKernelGGL(k_sort_scatter, dim3(kSortBlocks), dim3(256), 0, s, cur, L.q);
             order = L.q.sperm;
