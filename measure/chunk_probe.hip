@@ -35,7 +35,9 @@ __global__ __launch_bounds__(256) void k_chunk_probe(const float4* __restrict__ 
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = rec[k];
 #pragma unroll
-        for (int k = 0; k < K; k++) acc += v[k].x + v[k].w;
+        // every component is used, so each chunk stays one 16-byte load (a sum of fewer components
+        // lets the compiler load only those, as 4- and 8-byte loads straddling the chunks)
+        for (int k = 0; k < K; k++) acc += (v[k].x + v[k].y) + (v[k].z + v[k].w);
     }
     out[gid] = acc;  // keeps the loads live
 }
