@@ -87,14 +87,14 @@ def profiled_traffic(a):
     return int(m.group(1)) if m else None
 
 
-# Algorithmic bytes of k_shade (pt.hip): per shaded queue entry it reads the path state (radiance,
-# two pending contributions, throughput, pixel and sample keys: 72 B), the path and request ids
-# (8 B), the hit (16 B) and ray (32 B), three vertex indices (12 B), three shading normals (36 B),
-# two positions for the edge tangent (24 B) and the triangle's material id (4 B) = 204 B, and
-# writes 64 B of path state; each continuation ray it emits is 40 B (ray, path id, scene) and each
-# shadow ray 40 B (ray, slot, scene). Records shared by many paths (materials, lights, requests)
-# are cache-resident and not counted.
-SHADE_READ_B, SHADE_WRITE_B, SHADE_RAY_B = 204, 64, 40
+# Algorithmic bytes of k_shade (pt.hip): per shaded queue entry it reads the path's 64-byte record
+# (throughput, radiance, two pending contributions; pixel, sample and request ids in their w
+# lanes), the path id (4 B), the hit (16 B) and ray (32 B), three vertex indices (12 B), three
+# shading normals (36 B), two positions for the edge tangent (24 B) and the triangle's material id
+# (4 B) = 192 B, and writes the 64-byte record; each continuation ray it emits is 40 B (ray, path
+# id, scene) and each shadow ray 40 B (ray, slot, scene). Records shared by many paths (materials,
+# lights, requests) are cache-resident and not counted.
+SHADE_READ_B, SHADE_WRITE_B, SHADE_RAY_B = 192, 64, 40
 
 
 def shade_roofline(pt, cam, W, H, spp, depth, torch):

@@ -365,7 +365,7 @@ typedef struct sortgpu_server sortgpu_server;
 
 typedef struct sortgpu_server_opts {
     uint64_t hbm_budget;     /* bytes the server may hold: scenes + films + workspace (0 = 96 GiB) */
-    uint32_t wave_paths;     /* path slots per wave (0 = 33,554,432; 260 B of workspace each) */
+    uint32_t wave_paths;     /* path slots per wave (0 = 33,554,432; 248 B of workspace each) */
     uint32_t max_active;     /* requests admitted at once, 1..32 (0 = 32) */
     uint32_t inflight_waves; /* waves in flight at once, 1..4 (0 = 3), each on a stream and path
                                 workspace of its own, so their bounce loops overlap */

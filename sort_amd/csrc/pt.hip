@@ -74,6 +74,15 @@ __device__ __forceinline__ int chunk_of_path(const ChunkTable& ct, uint32_t p) {
 // Camera rays of every chunk (PerspectiveCamera::GenerateRay in closed form: raster -> camera
 // through the inverse of Scale(W,H,1) Scale(.5,-.5,1) Translate(1,-1,0) Perspective(xs, ys),
 // then the orthonormal ViewLookat basis).
+// A new path's record: throughput 1, nothing gathered, its pixel, sample and request.
+__device__ __forceinline__ void init_path(PathState ps, uint32_t p, uint32_t pix, uint32_t s, uint32_t req) {
+    float4* r = ps.rec + 4 * (size_t)p;
+    r[0] = make_float4(1.f, 1.f, 1.f, __uint_as_float(pix));
+    r[1] = make_float4(0.f, 0.f, 0.f, __uint_as_float(s));
+    r[2] = make_float4(0.f, 0.f, 0.f, __uint_as_float(req));
+    r[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ __launch_bounds__(kPtBlock) void k_raygen(const WaveDesc* __restrict__ w, PathState ps, Queues q) {
     const ChunkTable& ct = w->ct;
     const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
@@ -95,13 +104,7 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(const WaveDesc* __restrict_
     put_ray(q.ray[0], p, cam.eye, 0.0f, d, FLT_MAX);
     q.pid[0][p] = p;
     q.rsc[0][p] = R.scene;
-    ps.T[p] = make_float4(1.f, 1.f, 1.f, 0.f);
-    ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.c0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.c1[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.key[p] = pix;
-    ps.smp[p] = s;
-    ps.req[p] = c.req;
+    init_path(ps, p, pix, s, c.req);
 }
 
 // Caller-provided rays (the batched form of Integrator::Li) of request 0: keys[2i] = pixel,
@@ -115,13 +118,7 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen_rays(const float4* rays, co
     q.ray[0][2 * p + 1] = rays[2 * p + 1];
     q.pid[0][p] = p;
     q.rsc[0][p] = reqs[0].scene;
-    ps.T[p] = make_float4(1.f, 1.f, 1.f, 0.f);
-    ps.L[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.c0[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.c1[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    ps.key[p] = keys[2 * p];
-    ps.smp[p] = keys[2 * p + 1];
-    ps.req[p] = 0;
+    init_path(ps, p, keys[2 * p], keys[2 * p + 1], 0u);
 }
 
 __global__ void k_zero_counts(uint32_t* count, int which) {
@@ -137,7 +134,7 @@ __global__ __launch_bounds__(kPtBlock) void k_dump(uint32_t bounce, int cur, Pat
     const uint32_t n = q.count[cur];
     for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
         const uint32_t p = q.pid[cur][i];
-        const uint32_t pix = ps.key[p], s = ps.smp[p];
+        const uint32_t pix = __float_as_uint(ps.rec[4 * (size_t)p].w), s = __float_as_uint(ps.rec[4 * (size_t)p + 1].w);
         if (pix < d.pix_begin || pix >= d.pix_end || s >= d.spp || bounce >= d.depth) continue;
         const size_t k = ((size_t)(pix - d.pix_begin) * d.spp + s) * d.depth + bounce;
         d.rays[2 * k] = q.ray[cur][2 * i];
@@ -354,27 +351,26 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__
         uint32_t p = 0, scene = 0;
         if (i < n) {
             p = q.pid[cur][i];
-            const ReqDev& R = reqs[ps.req[p]];
+            float4* pr = ps.rec + 4 * (size_t)p;  // the path's record: one 64-byte read, one write
+            const float4 Tp = pr[0], Lp = pr[1], pc0 = pr[2], pc1 = pr[3];
+            const uint32_t req = __float_as_uint(pc0.w);
+            const ReqDev& R = reqs[req];
             scene = R.scene;
             const SceneDev& sc = scenes[scene];
             const uint32_t seed = R.seed, max_depth = R.max_depth;
             const float4 h = q.hit[i];
             const float4 r0 = q.ray[cur][2 * i], r1 = q.ray[cur][2 * i + 1];
             const V3 o = v3(r0.x, r0.y, r0.z), d = v3(r1.x, r1.y, r1.z);
-            float4 Lp = ps.L[p];
             V3 L = v3(Lp.x, Lp.y, Lp.z);
-            {   // fold the previous vertex's resolved NEE contributions
-                const float4 a = ps.c0[p], b = ps.c1[p];
-                L = L + v3(a.x, a.y, a.z) + v3(b.x, b.y, b.z);
-            }
+            // fold the previous vertex's resolved NEE contributions
+            L = L + v3(pc0.x, pc0.y, pc0.z) + v3(pc1.x, pc1.y, pc1.z);
             float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
-            const float4 Tp = ps.T[p];
             V3 T = v3(Tp.x, Tp.y, Tp.z);
             const uint32_t prim = __float_as_uint(h.y);
             if (prim == 0xFFFFFFFFu) {
                 if (bounce == 0 && sc.env_light >= 0) L = L + sc.env_L;  // Scene::Le on a camera miss
             } else {
-                const uint32_t key = ps.key[p], smp = ps.smp[p];
+                const uint32_t key = __float_as_uint(Tp.w), smp = __float_as_uint(Lp.w);
                 const Surf s = surface(sc, o, d, h.x, prim, h.z, h.w);
                 if (bounce == 0 && s.light >= 0) {  // SurfaceInteraction::Le -> AreaLight::Le
                     const LightRec& l = sc.lights[s.light];
@@ -479,10 +475,10 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(const SceneDev* __restrict__
                     }
                 }
             }
-            ps.L[p] = make_float4(L.x, L.y, L.z, 0.f);
-            ps.T[p] = make_float4(T.x, T.y, T.z, 0.f);
-            ps.c0[p] = c0;
-            ps.c1[p] = c1;
+            pr[0] = make_float4(T.x, T.y, T.z, Tp.w);
+            pr[1] = make_float4(L.x, L.y, L.z, Lp.w);
+            pr[2] = make_float4(c0.x, c0.y, c0.z, pc0.w);
+            pr[3] = c1;
         }
         // compaction: survivors and shadow rays append with one atomic per wave
         const uint32_t ci = wave_append(cont, &q.count[nxt]);
@@ -515,12 +511,10 @@ __global__ __launch_bounds__(kPtBlock) void k_resolve_att(PathState ps, Queues q
     const uint32_t n = q.count[2];
     for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
         const float4 a = q.sray[2 * (size_t)i];
-        const uint32_t slot = q.ssid[i];
-        float4* c = (slot & 1u) ? ps.c1 : ps.c0;
-        const float4 v = c[slot >> 1];
-        c[slot >> 1] = (a.x == 0.0f && a.y == 0.0f && a.z == 0.0f)
-                           ? make_float4(0.f, 0.f, 0.f, 0.f)
-                           : make_float4(v.x * a.x, v.y * a.y, v.z * a.z, 0.f);
+        float4* c = ps.rec + path_slot(q.ssid[i]);
+        const float4 v = *c;
+        *c = (a.x == 0.0f && a.y == 0.0f && a.z == 0.0f) ? make_float4(0.f, 0.f, 0.f, v.w)
+                                                         : make_float4(v.x * a.x, v.y * a.y, v.z * a.z, v.w);
     }
 }
 
@@ -542,7 +536,8 @@ __global__ __launch_bounds__(kPtBlock) void k_finalize(const WaveDesc* __restric
     float4 acc = film[pix];
     for (uint32_t s = 0; s < c.nsamp; s++) {
         const uint32_t p = c.path_base + s * c.npix + pix;
-        const float4 l = ps.L[p], a = ps.c0[p], b = ps.c1[p];
+        const float4* pr = ps.rec + 4 * (size_t)p;
+        const float4 l = pr[1], a = pr[2], b = pr[3];
         float r = l.x + a.x + b.x, gg = l.y + a.y + b.y, bb = l.z + a.z + b.z;
         if (cl > 0.0f) {  // Spectrum::Clamp(0, clamping): clamp() of math/utils.h:89-93, NaN kept
             r = r > cl ? cl : r;
@@ -566,7 +561,8 @@ __global__ __launch_bounds__(kPtBlock) void k_finalize(const WaveDesc* __restric
 __global__ __launch_bounds__(kPtBlock) void k_finalize_rays(PathState ps, uint32_t n, float* out) {
     const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
     if (p >= n) return;
-    const float4 l = ps.L[p], a = ps.c0[p], b = ps.c1[p];
+    const float4* pr = ps.rec + 4 * (size_t)p;
+    const float4 l = pr[1], a = pr[2], b = pr[3];
     out[3 * p] = l.x + a.x + b.x;
     out[3 * p + 1] = l.y + a.y + b.y;
     out[3 * p + 2] = l.z + a.z + b.z;
@@ -667,8 +663,7 @@ hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s) {
             // occluded shadow rays clear their contribution slots inside the any-hit launch
             sa.out = nullptr;
             sa.occ_slot = L.q.ssid;
-            sa.occ_c0 = L.ps.c0;
-            sa.occ_c1 = L.ps.c1;
+            sa.occ_rec = L.ps.rec;
             if (L.trees) sa.ray_scene = L.q.ssc;
             if ((e = launch_trace(sa, true, false, s, L.num_cus)) != hipSuccess) return e;
         }

@@ -91,7 +91,7 @@ PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_dep
 
 // Paths in flight per wave: about 16M, whole samples of the frame at a time (a frame of 16M
 // pixels or more runs one sample per wave). With two waves in flight, the workspace of a scene
-// object is at most 2 x 260 B per path slot: 8.7 GB for waves of 16M paths.
+// object is at most 2 x 248 B per path slot: 8.3 GB for waves of 16M paths.
 uint32_t samples_per_wave(uint32_t npix, uint32_t spp) {
     const uint64_t target = 1u << 24;
     uint64_t s = target / std::max<uint32_t>(npix, 1);

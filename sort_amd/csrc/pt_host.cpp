@@ -62,9 +62,7 @@ int PathWorkspace::ensure(sortgpu_ctx* ctx, uint32_t paths) {
     paths = std::max(paths, 1u);
     const size_t n = paths;
     int rc;
-    if ((rc = ws(mem, ctx, &ps.T, n)) || (rc = ws(mem, ctx, &ps.L, n)) || (rc = ws(mem, ctx, &ps.c0, n)) ||
-        (rc = ws(mem, ctx, &ps.c1, n)) || (rc = ws(mem, ctx, &ps.key, n)) || (rc = ws(mem, ctx, &ps.smp, n)) ||
-        (rc = ws(mem, ctx, &ps.req, n)) || (rc = ws(mem, ctx, &q.ray[0], 2 * n)) ||
+    if ((rc = ws(mem, ctx, &ps.rec, 4 * n)) || (rc = ws(mem, ctx, &q.ray[0], 2 * n)) ||
         (rc = ws(mem, ctx, &q.ray[1], 2 * n)) || (rc = ws(mem, ctx, &q.pid[0], n)) ||
         (rc = ws(mem, ctx, &q.pid[1], n)) || (rc = ws(mem, ctx, &q.rsc[0], n)) || (rc = ws(mem, ctx, &q.rsc[1], n)) ||
         (rc = ws(mem, ctx, &q.hit, n)) || (rc = ws(mem, ctx, &q.sray, 4 * n)) || (rc = ws(mem, ctx, &q.ssid, 2 * n)) ||

@@ -48,7 +48,7 @@ struct PtLaunch {
 };
 
 // Bytes of path workspace per path slot (state + queues + hit order), used for HBM budgeting.
-constexpr size_t kPathBytes = 4 * 16 + 3 * 4 + 2 * (32 + 4 + 4) + 16 + 2 * 32 + 2 * 4 + 2 * 4 + 2 * 4;
+constexpr size_t kPathBytes = 4 * 16 + 2 * (32 + 4 + 4) + 16 + 2 * 32 + 2 * 4 + 2 * 4 + 2 * 4;
 
 hipError_t pt_zero_f4(float4* a, uint32_t n, hipStream_t s);
 hipError_t pt_bounces(const PtLaunch& L, uint32_t npaths, hipStream_t s);

@@ -66,16 +66,18 @@ struct WaveDesc {
     ReqDev reqs[kMaxChunks];
 };
 
-// Path slots of one wave (fixed for the wave; index = path id).
+// Path slots of one wave (fixed for the wave; index = path id): one 64-byte record per path, so
+// the shading stage reads and writes a path's whole state in one 128-byte line (two paths to a
+// line) instead of gathering seven arrays. Record of path p, float4 index 4p + k:
+//   k = 0  throughput (xyz); w: pixel index (the RNG key), as bits
+//   k = 1  radiance so far (xyz); w: sample index (the RNG counter), as bits
+//   k = 2  pending light-sampled NEE contribution of the last vertex (xyz); w: request index, as bits
+//   k = 3  pending BSDF-sampled NEE contribution of the last vertex (xyz); w: 0
+// Shadow-ray slot s = 2p + j is the contribution at 4p + 2 + j; clearing or scaling one keeps its w.
 struct PathState {
-    float4* T;        // throughput
-    float4* L;        // radiance so far
-    float4* c0;       // pending light-sampled NEE contribution of the last vertex
-    float4* c1;       // pending BSDF-sampled NEE contribution of the last vertex
-    uint32_t* key;    // per path: pixel index (RNG key)
-    uint32_t* smp;    // per path: sample index (RNG counter)
-    uint32_t* req;    // per path: request index
+    float4* rec;
 };
+__host__ __device__ inline size_t path_slot(uint32_t s) { return 4 * (size_t)(s >> 1) + 2 + (s & 1u); }
 
 struct Queues {
     float4* ray[2];     // 2 x float4 per entry

@@ -383,10 +383,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             if (ANY) {
                 if (!a.occ_slot) {
                     static_cast<uint8_t*>(a.out)[ray_id] = occluded ? 1 : 0;
-                } else if (occluded) {  // the shadow ray's contribution slot is cleared here
+                } else if (occluded) {  // the shadow ray's contribution is cleared here (w kept)
                     const uint32_t slot = a.occ_slot[ray_id];
-                    float4* c = static_cast<float4*>((slot & 1u) ? a.occ_c1 : a.occ_c0);
-                    c[slot >> 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float* c = reinterpret_cast<float*>(a.occ_rec + 4 * (size_t)(slot >> 1) + 2 + (slot & 1u));
+                    c[0] = 0.f;
+                    c[1] = 0.f;
+                    c[2] = 0.f;
                 }
             } else {
                 reinterpret_cast<float4*>(a.out)[ray_id] = make_float4(bt, __uint_as_float(bp), bu, bv);

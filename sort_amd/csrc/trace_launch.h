@@ -40,11 +40,11 @@ struct TraceArgs {
     // (0 or 1: packed; 2: over the first half of ray i's own 32-byte record when out == rays)
     uint32_t out_stride = 0;
     // any-hit only, for the path tracer's shadow rays: with occ_slot set, an occluded ray i clears
-    // the 16-byte contribution slot occ_slot[i] (an odd slot s in occ_c1, an even one in occ_c0,
-    // at index s / 2) and nothing is written to out; an unoccluded ray writes nothing
+    // the colour (x, y, z) of its path's contribution slot, float4 occ_rec[4 (s / 2) + 2 + s % 2]
+    // for s = occ_slot[i], keeping its w, and nothing is written to out; an unoccluded ray writes
+    // nothing
     const uint32_t* occ_slot = nullptr;
-    void* occ_c0 = nullptr;
-    void* occ_c1 = nullptr;
+    float4* occ_rec = nullptr;
 };
 
 // Launches the closest (any = false) or any-hit traversal on `stream`.
