@@ -20,9 +20,10 @@ cpu_baseline: the oracle (C++ AVX restatement of SORT's Obvh traversal, std::thr
 chunks) on the same incoherent rays, on this host's cores.
 
 Further legs, reported beside the headline under their own keys (--legs selects them):
-  pt       -- the wavefront path tracer: the Cornell box (256^2, 16 spp, depth 5) and the 2M-triangle
-              spheres scene (1024^2, 64 spp, depth 8), samples/s and Mrays/s, with the oracle's CPU
-              path tracer timed on a bounded crop of each;
+  pt       -- the wavefront path tracer: the Cornell box (256^2, 16 spp, depth 5, OBVH) and the
+              2M-triangle spheres scene (1024^2, 64 spp, depth 8, node width picked by the
+              autotuner), samples/s and Mrays/s (median of 3 frames after one at full size), with the
+              oracle's CPU path tracer timed on a bounded crop of each;
   batched  -- 64 scenes in one concatenated pool, one persistent launch over all their incoherent
               rays, against one launch per scene;
   served   -- the render server under Poisson arrivals (20 requests/s over 16 resident scenes) and
@@ -116,23 +117,49 @@ def shade_roofline(pt, cam, W, H, spp, depth, torch):
             "note": "launch times on the render's two overlapping streams (shared GPU time included)"}
 
 
-def leg_pt(ctx, cpu=True):
-    """Path tracer throughput on the Cornell box and on the 2M-triangle spheres scene."""
-    import torch
-
-    from sort_amd import Obvh, integrator as I, scenes
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    out = {}
-    cases = [("cornell", scenes.cornell_sphere(), 256, 256, 16, 5, 3),
-             ("spheres", scenes.spheres_scene(), 1024, 1024, 64, 8, 1)]
-    for name, sc, W, H, spp, depth, reps in cases:
-        t = time.perf_counter()
-        acc = Obvh(ctx).build(sc["pos"], sc["idx"], sc["quads"])
+def pt_width(ctx, sc, cam, W, H, spp, depth):
+    """Node width for a path-traced frame, picked by the autotuner (autotune.tune over the two
+    widths, default knobs otherwise) on a sample render of the frame at 1/16 of its samples (4M
+    paths for the spheres frame). The images do not depend on the width. Returns (width,
+    {width: ms})."""
+    from sort_amd import Obvh, Qbvh, autotune, integrator as I
+    runners, keep = {}, []
+    for w, cls in ((8, Obvh), (4, Qbvh)):
+        acc = cls(ctx).build(sc["pos"], sc["idx"], sc["quads"])
         pt = I.PathTracing(acc, sc["materials"], tri_material=sc.get("tri_material"), normals=sc.get("normals"),
                            quad_radiance=sc["quad_radiance"], env_radiance=sc["env"])
+        keep.append((acc, pt))
+        runners[w] = lambda pt=pt: pt.render(cam, W, H, max(1, spp // 16), seed=7, max_depth=depth)
+    best, results = autotune.tune(ctx, runners, lds=(0,), refills=(0,), blocks=(0,), leaf_batches=(0,), leaf_mins=(0,),
+                                  reps=3)
+    for acc, pt in keep:
+        pt.close()
+        acc.release()
+    return int(best["width"]), {int(r["width"]): round(r["ms"], 2) for r in results}
+
+
+def leg_pt(ctx, cpu=True):
+    """Path tracer throughput on the Cornell box (OBVH, as its configuration names) and on the
+    2M-triangle spheres scene (node width picked by the autotuner). Each frame is rendered once at
+    its full size before the timed frames, so the path workspace is sized outside the timing."""
+    import torch
+
+    from sort_amd import Obvh, Qbvh, integrator as I, scenes
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    out = {}
+    cases = [("cornell", scenes.cornell_sphere(), 256, 256, 16, 5, 3, 8),
+             ("spheres", scenes.spheres_scene(), 1024, 1024, 64, 8, 3, 0)]
+    for name, sc, W, H, spp, depth, reps, width in cases:
         cam = I.camera(**sc["camera"])
+        width_ms = None
+        if width == 0:
+            width, width_ms = pt_width(ctx, sc, cam, W, H, spp, depth)
+        t = time.perf_counter()
+        acc = (Obvh if width == 8 else Qbvh)(ctx).build(sc["pos"], sc["idx"], sc["quads"])
+        pt = I.PathTracing(acc, sc["materials"], tri_material=sc.get("tri_material"), normals=sc.get("normals"),
+                           quad_radiance=sc["quad_radiance"], env_radiance=sc["env"])
         build_s = time.perf_counter() - t
-        pt.render(cam, W, H, 1, seed=99, max_depth=depth)
+        pt.render(cam, W, H, spp, seed=99, max_depth=depth)
         torch.cuda.synchronize()
         secs, ext, shadow = [], 0, 0
         for r in range(reps):
@@ -145,10 +172,12 @@ def leg_pt(ctx, cpu=True):
             shadow += st["shadow_rays"]
         sec = float(np.median(secs))
         paths = W * H * spp
-        res = {"size": f"{W}x{H}", "spp": spp, "depth": depth, "tris": int(len(sc["idx"])),
+        res = {"size": f"{W}x{H}", "spp": spp, "depth": depth, "tris": int(len(sc["idx"])), "width": width,
                "ms": round(sec * 1e3, 2), "msamples_s": round(paths / sec / 1e6, 2),
                "mrays_s": round((ext + shadow) / reps / sec / 1e6, 2),
                "rays_per_path": round((ext + shadow) / reps / paths, 3), "build_s": round(build_s, 2)}
+        if width_ms:
+            res["width_sample_ms"] = width_ms
         res["shade_roofline"] = shade_roofline(pt, cam, W, H, spp, depth, torch)
         if cpu:
             from oracle.oracle import OraclePT
