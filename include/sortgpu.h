@@ -225,7 +225,7 @@ int sortgpu_accel_errors(sortgpu_accel* accel, void* stream, uint32_t* flags);
  * leaf_min = a leaf-testing phase ends when fewer lanes hold a leaf (default 1, 1..64);
  * reorder = traversal order of sortgpu_trace_closest / _any / _attenuation / _multi and
  * sortgpu_pool_trace batches: 0 (default) sorts batches of at least 262,144 rays by origin cell
- * (64^3 grid over the scene box; pooled rays by scene, then by cell in their scene's box) before the
+ * (128^3 grid over the scene box; pooled rays by scene, then by cell in their scene's box) before the
  * launch (stream-ordered device memory of 16 bytes per ray plus sort workspace, from a pool the
  * context owns; a batch sharing one origin, such as camera rays, keeps its order and pays only the
  * sort), 1 = never, 2 = always; shade_order = order in which the path tracer shades each bounce's

@@ -113,3 +113,40 @@ def test_pool_traversal_order_changes_no_result(gpu_ctx, work8, any_hit):
     with pytest.raises(Exception, match="outside the pool"):
         pool.check_errors()
     pool.close()
+
+
+def test_pool_of_many_scenes_orders_on_a_coarser_grid(gpu_ctx):
+    """A pool of more scenes than the default key leaves room for (2,100 ids need 12 bits, so the
+    origin cell drops to a 64^3 grid) is still taken in (scene, cell) order, and every ray's hit
+    equals index order's and its own scene's accelerator's."""
+    from sort_amd import Obvh, scenes
+    n_scenes = 2100
+    g = np.random.default_rng(2100)
+    accels = []
+    print(f"[batch] building {n_scenes} tiny scenes", flush=True)
+    for i in range(n_scenes):
+        c = g.random((6, 1, 3)).astype(np.float32)
+        pos = (c + np.float32(0.3) * g.normal(size=(6, 3, 3)).astype(np.float32)).reshape(-1, 3)
+        accels.append(Obvh(gpu_ctx).build(np.ascontiguousarray(pos), np.arange(18, dtype=np.uint32).reshape(6, 3)))
+    pool = batch.Pool(gpu_ctx, accels)
+    rays = scenes.random_rays(1 << 14, 77, lo=0.0, hi=1.0)
+    sid = g.integers(0, n_scenes, len(rays)).astype(np.uint32)
+    d_rays = torch.from_numpy(rays).cuda()
+    d_sid = torch.from_numpy(sid.view(np.int32)).cuda()
+    try:
+        batch.set_tuning(gpu_ctx, reorder=1)
+        plain = pool.trace(d_rays, d_sid).cpu().numpy()
+        batch.set_tuning(gpu_ctx, reorder=2)
+        ordered = pool.trace(d_rays, d_sid).cpu().numpy()
+    finally:
+        batch.set_tuning(gpu_ctx)
+    assert np.array_equal(plain.view(np.uint32), ordered.view(np.uint32))
+    for i in range(0, n_scenes, 97):
+        m = sid == i
+        if m.any():
+            own = accels[i].get_intersect(d_rays[torch.from_numpy(np.nonzero(m)[0]).cuda()]).cpu().numpy()
+            assert np.array_equal(own.view(np.uint32), plain[m].view(np.uint32))
+    assert np.any(plain[:, 1].view(np.uint32) != MISS)
+    pool.close()
+    for a in accels:
+        a.release()
