@@ -18,10 +18,10 @@ struct sortgpu_scene {
     SceneDev host{};              // pointers are device addresses
     SceneDev* d_scene = nullptr;  // one-entry scene table
     DeviceAllocs mem;             // shading data + scene table
-    // A frame's waves alternate over two internal streams, each with its own workspace, so one
-    // wave's latency-bound last bounces overlap the next wave's first; the caller's stream waits
-    // for both at the end, and film updates run in wave order (events).
-    static constexpr int kStreams = 2;
+    // A frame's waves take turns on three internal streams, each with its own workspace, so the
+    // latency-bound last bounces of some waves overlap the first bounces of others; the caller's
+    // stream waits for all of them at the end, and film updates run in wave order (events).
+    static constexpr int kStreams = 3;
     PathWorkspace ws[kStreams];
     hipStream_t rs[kStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_film = nullptr, ev_join[kStreams] = {};
@@ -89,11 +89,11 @@ PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_dep
     return L;
 }
 
-// Paths in flight per wave: about 16M, whole samples of the frame at a time (a frame of 16M
-// pixels or more runs one sample per wave). With two waves in flight, the workspace of a scene
-// object is at most 2 x 248 B per path slot: 8.3 GB for waves of 16M paths.
+// Paths in flight per wave: about 8M, whole samples of the frame at a time (a frame of 8M
+// pixels or more runs one sample per wave). With three waves in flight, the workspace of a scene
+// object is at most 3 x 248 B per path slot: 6.2 GB for waves of 8M paths.
 uint32_t samples_per_wave(uint32_t npix, uint32_t spp) {
-    const uint64_t target = 1u << 24;
+    const uint64_t target = 1u << 23;
     uint64_t s = target / std::max<uint32_t>(npix, 1);
     s = std::max<uint64_t>(1, std::min<uint64_t>(s, spp));
     return (uint32_t)s;
