@@ -90,11 +90,16 @@ PtLaunch make_launch(sortgpu_scene* sc, const WaveDesc* d_wave, uint32_t max_dep
 }
 
 // Paths in flight per wave: about 8M, whole samples of the frame at a time (a frame of 8M
-// pixels or more runs one sample per wave). With three waves in flight, the workspace of a scene
-// object is at most 3 x 248 B per path slot: 6.2 GB for waves of 8M paths.
+// pixels or more runs one sample per wave), and at least one wave per internal stream as long as
+// each wave keeps 2^18 paths or more (a 1M-path frame runs as three waves whose thin last bounces
+// overlap). With three waves in flight, the workspace of a scene object is at most 3 x 248 B per
+// path slot: 6.2 GB for waves of 8M paths.
 uint32_t samples_per_wave(uint32_t npix, uint32_t spp) {
-    const uint64_t target = 1u << 23;
-    uint64_t s = target / std::max<uint32_t>(npix, 1);
+    const uint64_t target = 1u << 23, min_paths = 1u << 18;
+    const uint64_t np = std::max<uint32_t>(npix, 1);
+    uint64_t s = target / np;
+    s = std::min<uint64_t>(s, (spp + sortgpu_scene::kStreams - 1) / sortgpu_scene::kStreams);
+    s = std::max<uint64_t>(s, (min_paths + np - 1) / np);
     s = std::max<uint64_t>(1, std::min<uint64_t>(s, spp));
     return (uint32_t)s;
 }
