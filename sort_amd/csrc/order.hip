@@ -224,11 +224,4 @@ hipError_t order_rays_scratch(void* base, const void* rays, uint32_t n, const fl
     return hipSuccess;
 }
 
-size_t order_scratch_temp_bytes(uint32_t n) {
-    hipcub::DoubleBuffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
-    size_t tb = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, (int)n, 0, kKeyBits, nullptr) != hipSuccess) return 0;
-    return tb;
-}
-
 }  // namespace sortgpu

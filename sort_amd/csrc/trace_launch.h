@@ -96,8 +96,6 @@ hipError_t scratch_base(int device, void** out);
 // order_rays on the array at `base` (n <= kScratchRays); *order points into it.
 hipError_t order_rays_scratch(void* base, const void* rays, uint32_t n, const float bbox[6], hipStream_t s,
                               const uint32_t** order);
-// Radix-sort workspace bytes a batch of n rays needs (0 if the query fails).
-size_t order_scratch_temp_bytes(uint32_t n);
 // Error flags the traversal kernels raise (atomicOr into a 16-byte device word block).
 constexpr uint32_t kFlagStackOverflow = 1u;  // a push found the stack full: that ray's result is incomplete
 constexpr uint32_t kFlagBadScene = 2u;       // multi-scene form: a ray named a tree outside the table
