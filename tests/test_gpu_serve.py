@@ -133,9 +133,9 @@ def test_eviction_and_cancellation_leave_results_unchanged(gpu_ctx, world):
 
 
 def test_multi_wave_render_equals_served(gpu_ctx, world):
-    """A frame too big for one wave of the single-request render (1024^2 at 8 spp: four waves of two
-    samples, alternating over two streams) equals, bit for bit, the server's result for the same
-    request (one 8-sample wave)."""
+    """A frame the single-request render splits into waves (1024^2 at 8 spp: three waves of 3, 3 and
+    2 samples on its three streams) equals, bit for bit, the server's result for the same request
+    (one 8-sample wave)."""
     cam = I.camera(**world[0][1]["camera"])
     plain = _plain(world, 0, cam, 1024, 1024, 8, 5, 77)
     srv, ids = _server(gpu_ctx, world)
