@@ -22,8 +22,9 @@ chunks) on the same incoherent rays, on this host's cores.
 Further legs, reported beside the headline under their own keys (--legs selects them):
   pt       -- the wavefront path tracer: the Cornell box (256^2, 16 spp, depth 5, OBVH) and the
               2M-triangle spheres scene (1024^2, 64 spp, depth 8, node width picked by the
-              autotuner), samples/s and Mrays/s (median of 3 frames after one at full size), with the
-              oracle's CPU path tracer timed on a bounded crop of each;
+              autotuner), refill threshold picked by the autotuner per frame (the default-knob
+              rate beside it), samples/s and Mrays/s (median of 3 frames after one at full size),
+              with the oracle's CPU path tracer timed on a bounded crop of each;
   batched  -- 64 scenes in one concatenated pool, one persistent launch over all their incoherent
               rays, against one launch per scene;
   served   -- the render server under Poisson arrivals (20 requests/s over 16 resident scenes) and
@@ -114,46 +115,51 @@ def shade_roofline(pt, cam, W, H, spp, depth, torch):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": "k_shade", "bytes": int(b),
             "entries": int(ext), "launches": tm["bounces"], "kernel_ms": round(tm["shade_ms"], 2),
             "extend_ms": round(tm["extend_ms"], 2), "shadow_ms": round(tm["shadow_ms"], 2),
-            "note": "launch times on the render's two overlapping streams (shared GPU time included)"}
+            "note": "launch times on the render's overlapping internal streams (shared GPU time included)"}
 
 
-def pt_width(ctx, sc, cam, W, H, spp, depth):
-    """Node width for a path-traced frame, picked by the autotuner (autotune.tune over the two
-    widths, default knobs otherwise) on a sample render of the frame at 1/16 of its samples (4M
-    paths for the spheres frame). The images do not depend on the width. Returns (width,
-    {width: ms})."""
+def pt_tune(ctx, sc, cam, W, H, spp, depth, widths):
+    """Node width and refill threshold for a path-traced frame, picked by the autotuner
+    (autotune.tune over `widths` and refill 16 / 32 / 48, other knobs at their defaults) on a
+    sample render of the frame: 1/16 of its samples, or the whole frame when that is under 2^20
+    paths (the Cornell frame). The images do not depend on either knob. Returns (best tuning,
+    {"w<width>/r<refill>": ms})."""
     from sort_amd import Obvh, Qbvh, autotune, integrator as I
+    sspp = max(1, spp // 16)
+    if W * H * sspp < (1 << 20):
+        sspp = spp
     runners, keep = {}, []
-    for w, cls in ((8, Obvh), (4, Qbvh)):
-        acc = cls(ctx).build(sc["pos"], sc["idx"], sc["quads"])
+    for w in widths:
+        acc = (Obvh if w == 8 else Qbvh)(ctx).build(sc["pos"], sc["idx"], sc["quads"])
         pt = I.PathTracing(acc, sc["materials"], tri_material=sc.get("tri_material"), normals=sc.get("normals"),
                            quad_radiance=sc["quad_radiance"], env_radiance=sc["env"])
         keep.append((acc, pt))
-        runners[w] = lambda pt=pt: pt.render(cam, W, H, max(1, spp // 16), seed=7, max_depth=depth)
-    best, results = autotune.tune(ctx, runners, lds=(0,), refills=(0,), blocks=(0,), leaf_batches=(0,), leaf_mins=(0,),
-                                  reps=3)
+        runners[w] = lambda pt=pt: pt.render(cam, W, H, sspp, seed=7, max_depth=depth)
+    best, results = autotune.tune(ctx, runners, lds=(0,), refills=(16, 32, 48), blocks=(0,), leaf_batches=(0,),
+                                  leaf_mins=(0,), reps=3)
     for acc, pt in keep:
         pt.close()
         acc.release()
-    return int(best["width"]), {int(r["width"]): round(r["ms"], 2) for r in results}
+    return best, {f"w{r['width']}/r{r['refill']}": round(r["ms"], 3) for r in results}
 
 
 def leg_pt(ctx, cpu=True):
     """Path tracer throughput on the Cornell box (OBVH, as its configuration names) and on the
-    2M-triangle spheres scene (node width picked by the autotuner). Each frame is rendered once at
-    its full size before the timed frames, so the path workspace is sized outside the timing."""
+    2M-triangle spheres scene (node width picked by the autotuner), with the refill threshold the
+    autotuner picks for each frame (the rate with the default knobs is reported beside it). Each
+    frame is rendered once at its full size before the timed frames, so the path workspace is
+    sized outside the timing."""
     import torch
 
-    from sort_amd import Obvh, Qbvh, integrator as I, scenes
+    from sort_amd import Obvh, Qbvh, batch, integrator as I, scenes
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     out = {}
-    cases = [("cornell", scenes.cornell_sphere(), 256, 256, 16, 5, 3, 8),
-             ("spheres", scenes.spheres_scene(), 1024, 1024, 64, 8, 3, 0)]
-    for name, sc, W, H, spp, depth, reps, width in cases:
+    cases = [("cornell", scenes.cornell_sphere(), 256, 256, 16, 5, 3, (8,)),
+             ("spheres", scenes.spheres_scene(), 1024, 1024, 64, 8, 3, (8, 4))]
+    for name, sc, W, H, spp, depth, reps, widths in cases:
         cam = I.camera(**sc["camera"])
-        width_ms = None
-        if width == 0:
-            width, width_ms = pt_width(ctx, sc, cam, W, H, spp, depth)
+        best, tune_ms = pt_tune(ctx, sc, cam, W, H, spp, depth, widths)  # leaves `best` applied
+        width = int(best["width"])
         t = time.perf_counter()
         acc = (Obvh if width == 8 else Qbvh)(ctx).build(sc["pos"], sc["idx"], sc["quads"])
         pt = I.PathTracing(acc, sc["materials"], tri_material=sc.get("tri_material"), normals=sc.get("normals"),
@@ -176,9 +182,17 @@ def leg_pt(ctx, cpu=True):
                "ms": round(sec * 1e3, 2), "msamples_s": round(paths / sec / 1e6, 2),
                "mrays_s": round((ext + shadow) / reps / sec / 1e6, 2),
                "rays_per_path": round((ext + shadow) / reps / paths, 3), "build_s": round(build_s, 2)}
-        if width_ms:
-            res["width_sample_ms"] = width_ms
+        res["tuning"] = {"width": width, "refill": int(best["refill"])}
+        res["tune_sample_ms"] = tune_ms
         res["shade_roofline"] = shade_roofline(pt, cam, W, H, spp, depth, torch)
+        batch.set_tuning(ctx)  # the same frame with the default knobs on the same tree
+        dsecs = []
+        for r in range(reps):
+            t = time.perf_counter()
+            pt.render(cam, W, H, spp, seed=1 + r, max_depth=depth)
+            torch.cuda.synchronize()
+            dsecs.append(time.perf_counter() - t)
+        res["default_knobs_msamples_s"] = round(paths / float(np.median(dsecs)) / 1e6, 2)
         if cpu:
             from oracle.oracle import OraclePT
             o = OraclePT(sc)
