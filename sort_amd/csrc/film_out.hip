@@ -61,13 +61,29 @@ __host__ __device__ inline uint16_t exr_half(float f) {
     return (uint16_t)((o & 0x7fffu) | (sign << 15));
 }
 
-// One thread per pixel: scanline y holds B[w], G[w], R[w] halves.
-__global__ void k_exr_lines(const float* __restrict__ rgb, uint32_t w, uint32_t h, uint16_t* __restrict__ out) {
+// The file image of a device film from its offset table on (the header is written on the host):
+// body[0, 8h) the offset table, then one chunk per scanline, (y, byte count, B[w], G[w], R[w]).
+// One thread per pixel writes its three halves; the first thread of a line also writes the line's
+// chunk header and offset-table entry (file offset = head + body offset). A chunk is 8 + 6w bytes,
+// an even count, so every half sits at an even body offset; the chunk header's 32-bit words need
+// not be 4-byte aligned and are written byte by byte.
+__global__ void k_exr_body(const float* __restrict__ rgb, uint32_t w, uint32_t h, uint64_t head,
+                           uint8_t* __restrict__ body) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)w * h) return;
     const uint64_t y = i / w, x = i - y * w;
+    const uint64_t chunk = 8ull + 6ull * w;
+    uint8_t* c = body + 8ull * h + y * chunk;
+    if (x == 0) {
+        reinterpret_cast<uint64_t*>(body)[y] = head + 8ull * h + y * chunk;
+        const uint32_t yy = (uint32_t)y, len = 6u * w;
+        for (int b = 0; b < 4; b++) {
+            c[b] = (uint8_t)(yy >> (8 * b));
+            c[4 + b] = (uint8_t)(len >> (8 * b));
+        }
+    }
     const float* p = rgb + 3 * i;
-    uint16_t* line = out + 3 * (uint64_t)w * y;
+    uint16_t* line = reinterpret_cast<uint16_t*>(c + 8);
     line[x] = exr_half(p[2]);
     line[w + x] = exr_half(p[1]);
     line[2 * (uint64_t)w + x] = exr_half(p[0]);
@@ -159,8 +175,8 @@ int check_size(uint32_t w, uint32_t h, const char* what) {
     return SORTGPU_OK;
 }
 
-// The half scanlines of a film in host memory (from a device film: converted on the device).
-int film_lines(const float* film, int on_device, uint32_t w, uint32_t h, std::vector<uint16_t>& lines, void* stream) {
+// The half scanlines of a film in host memory.
+int film_lines_host(const float* film, uint32_t w, uint32_t h, std::vector<uint16_t>& lines) {
     const uint64_t n = (uint64_t)w * h;
     try {
         lines.resize(3 * n);
@@ -168,26 +184,34 @@ int film_lines(const float* film, int on_device, uint32_t w, uint32_t h, std::ve
         set_error(nullptr, "film output: host allocation failed");
         return SORTGPU_ERR_OOM;
     }
-    if (!on_device) {
-        for (uint64_t i = 0; i < n; i++) {
-            const uint64_t y = i / w, x = i - y * w;
-            uint16_t* line = lines.data() + 3 * (uint64_t)w * y;
-            line[x] = exr_half(film[3 * i + 2]);
-            line[w + x] = exr_half(film[3 * i + 1]);
-            line[2 * (uint64_t)w + x] = exr_half(film[3 * i + 0]);
-        }
-        return SORTGPU_OK;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t y = i / w, x = i - y * w;
+        uint16_t* line = lines.data() + 3 * (uint64_t)w * y;
+        line[x] = exr_half(film[3 * i + 2]);
+        line[w + x] = exr_half(film[3 * i + 1]);
+        line[2 * (uint64_t)w + x] = exr_half(film[3 * i + 0]);
     }
+    return SORTGPU_OK;
+}
+
+// The EXR file of a device film into `out`: the file image after the header is built on the
+// device and copied into place in one transfer; the host writes only the header.
+int film_exr_device(const float* film, uint32_t w, uint32_t h, uint8_t* out, void* stream) {
+    const uint64_t n = (uint64_t)w * h;
+    const std::vector<uint8_t> head = exr_header(w, h);
+    const uint64_t body_bytes = 8ull * h + (8ull + 6ull * w) * h;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    uint16_t* d = nullptr;
-    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&d), 6 * n, s);
+    uint8_t* d = nullptr;
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&d), body_bytes, s);
     if (e != hipSuccess) return hip_fail(nullptr, e, "film output: hipMallocAsync");
     const uint32_t block = 256;
-    hipLaunchKernelGGL(k_exr_lines, dim3((uint32_t)((n + block - 1) / block)), dim3(block), 0, s, film, w, h, d);
+    hipLaunchKernelGGL(k_exr_body, dim3((uint32_t)((n + block - 1) / block)), dim3(block), 0, s, film, w, h,
+                       (uint64_t)head.size(), d);
     e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(lines.data(), d, 6 * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out + head.size(), d, body_bytes, hipMemcpyDeviceToHost, s);
     const hipError_t ef = hipFreeAsync(d, s);
     if (e == hipSuccess) e = ef;
+    std::memcpy(out, head.data(), head.size());
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(nullptr, e, "film output");
     return SORTGPU_OK;
@@ -231,8 +255,9 @@ int sortgpu_film_encode_exr(const float* film_rgb, int on_device, uint32_t width
                   (unsigned long long)exr_bytes(width, height), (unsigned long long)cap);
         return SORTGPU_ERR_ARG;
     }
+    if (on_device) return film_exr_device(film_rgb, width, height, out, stream);
     std::vector<uint16_t> lines;
-    rc = film_lines(film_rgb, on_device, width, height, lines, stream);
+    rc = film_lines_host(film_rgb, width, height, lines);
     if (rc != SORTGPU_OK) return rc;
     exr_assemble(width, height, lines.data(), out);
     return SORTGPU_OK;
