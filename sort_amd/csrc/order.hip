@@ -5,9 +5,9 @@
 // at the same time share those reads. Incoherent batches (bounce rays from random surface points)
 // arrive in no useful order, so before the launch the batch is sorted (stably) by its origin's
 // cell on a 128^3 grid over the tree's box, in Morton order (a 21-bit key: the same three 8-bit
-// radix passes as 64^3, and a finer order that measured 0.6% faster on incoherent batches). Direction cells in
-// the key were measured to cost camera batches 4% and gain bounce batches under 1%, so the key
-// has none: a batch sharing one origin keeps its order. The kernel takes rays in the sorted
+// radix passes as a 64^3 grid, and an order that measured 0.6% faster on incoherent batches).
+// Direction cells or octants in the key were measured to cost camera batches 4-5% and gain bounce
+// batches under 1%, so the key has none: a batch sharing one origin keeps its order. The kernel takes rays in the sorted
 // order; each ray's result is written at its own index and does not depend on the order, so
 // results are bit-identical with or without it.
 #include <hip/hip_runtime.h>

@@ -71,9 +71,6 @@ __device__ __forceinline__ int chunk_of_path(const ChunkTable& ct, uint32_t p) {
     return k;
 }
 
-// Camera rays of every chunk (PerspectiveCamera::GenerateRay in closed form: raster -> camera
-// through the inverse of Scale(W,H,1) Scale(.5,-.5,1) Translate(1,-1,0) Perspective(xs, ys),
-// then the orthonormal ViewLookat basis).
 // A new path's record: throughput 1, nothing gathered, its pixel, sample and request.
 __device__ __forceinline__ void init_path(PathState ps, uint32_t p, uint32_t pix, uint32_t s, uint32_t req) {
     float4* r = ps.rec + 4 * (size_t)p;
@@ -83,6 +80,9 @@ __device__ __forceinline__ void init_path(PathState ps, uint32_t p, uint32_t pix
     r[3] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Camera rays of every chunk (PerspectiveCamera::GenerateRay in closed form: raster -> camera
+// through the inverse of Scale(W,H,1) Scale(.5,-.5,1) Translate(1,-1,0) Perspective(xs, ys),
+// then the orthonormal ViewLookat basis).
 __global__ __launch_bounds__(kPtBlock) void k_raygen(const WaveDesc* __restrict__ w, PathState ps, Queues q) {
     const ChunkTable& ct = w->ct;
     const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
@@ -134,7 +134,8 @@ __global__ __launch_bounds__(kPtBlock) void k_dump(uint32_t bounce, int cur, Pat
     const uint32_t n = q.count[cur];
     for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += gridDim.x * kPtBlock) {
         const uint32_t p = q.pid[cur][i];
-        const uint32_t pix = __float_as_uint(ps.rec[4 * (size_t)p].w), s = __float_as_uint(ps.rec[4 * (size_t)p + 1].w);
+        const float4* pr = ps.rec + 4 * (size_t)p;
+        const uint32_t pix = __float_as_uint(pr[0].w), s = __float_as_uint(pr[1].w);
         if (pix < d.pix_begin || pix >= d.pix_end || s >= d.spp || bounce >= d.depth) continue;
         const size_t k = ((size_t)(pix - d.pix_begin) * d.spp + s) * d.depth + bounce;
         d.rays[2 * k] = q.ray[cur][2 * i];
