@@ -63,6 +63,9 @@ def parse():
                    help="node width of the served scenes' trees (0: the autotuner's pick on a sample of the renders)")
     p.add_argument("--shade-order", type=int, default=0, choices=[0, 1, 2],
                    help="path tracer: shade each bounce's hits in spatial order (0 default, 1 never, 2 always)")
+    p.add_argument("--pt-no-tune", action="store_true",
+                   help="pt leg: no autotuning (Cornell on its OBVH, spheres on the 4-wide tree, default knobs), "
+                        "so that a profile holds only the measured frames")
     p.add_argument("--copy-probe", action="store_true",
                    help="end with three 256 MiB device copies (HBM counter calibration)")
     return p.parse_args()
@@ -143,7 +146,7 @@ def pt_tune(ctx, sc, cam, W, H, spp, depth, widths):
     return best, {f"w{r['width']}/r{r['refill']}": round(r["ms"], 3) for r in results}
 
 
-def leg_pt(ctx, cpu=True):
+def leg_pt(ctx, cpu=True, tune=True):
     """Path tracer throughput on the Cornell box (OBVH, as its configuration names) and on the
     2M-triangle spheres scene (node width picked by the autotuner), with the refill threshold the
     autotuner picks for each frame (the rate with the default knobs is reported beside it). Each
@@ -158,7 +161,10 @@ def leg_pt(ctx, cpu=True):
              ("spheres", scenes.spheres_scene(), 1024, 1024, 64, 8, 3, (8, 4))]
     for name, sc, W, H, spp, depth, reps, widths in cases:
         cam = I.camera(**sc["camera"])
-        best, tune_ms = pt_tune(ctx, sc, cam, W, H, spp, depth, widths)  # leaves `best` applied
+        if tune:
+            best, tune_ms = pt_tune(ctx, sc, cam, W, H, spp, depth, widths)  # leaves `best` applied
+        else:
+            best, tune_ms = {"width": widths[-1], "refill": 16}, {}
         width = int(best["width"])
         t = time.perf_counter()
         acc = (Obvh if width == 8 else Qbvh)(ctx).build(sc["pos"], sc["idx"], sc["quads"])
@@ -618,7 +624,7 @@ def main():
     from sort_amd.batch import set_tuning
     set_tuning(ctx, shade_order=a.shade_order)
     if "pt" in legs:
-        out["pt"] = leg_pt(ctx, cpu=not a.no_cpu)
+        out["pt"] = leg_pt(ctx, cpu=not a.no_cpu, tune=not a.pt_no_tune)
     if "batched" in legs:
         out["batched"] = leg_batched(ctx)
         set_tuning(ctx, shade_order=a.shade_order)  # the batched leg leaves the defaults applied
